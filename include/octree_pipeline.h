@@ -1,0 +1,234 @@
+/*
+ * octree_pipeline.h - C ABI of the MI355X (gfx950) octree-construction
+ * pipeline: Morton encode, OneSweep radix sort, unique, Karras radix tree,
+ * edge count, prefix sum, octree nodes.
+ *
+ * Conventions
+ *  - Every function returns an ocp_status; 0 is success. The text of the last
+ *    failure is ocp_last_error(ctx). Nothing exits or throws across the ABI.
+ *  - All array arguments are device pointers (allocate with ocp_malloc or any
+ *    hipMalloc'd memory of the same process). Counts that the reference
+ *    passes by value are host integers.
+ *  - Work is enqueued asynchronously on the context's stream; ocp_synchronize
+ *    waits for it. One context per host thread; contexts are not locked.
+ *  - Layouts are the reference's: float4 points (16 B, w ignored) or the
+ *    declared float3 variant (12 B), uint32 keys, uint8 prefix lengths,
+ *    uint8 booleans, int32 indices, 60-byte OctNode.
+ *  - Results are bit-identical to the reference CPU path (kernels/0N_*.cpp of
+ *    ucsc-redwood/pipeline) run single-threaded in ascending index order on
+ *    zeroed buffers.
+ *
+ * Per-stage entry points replace the reference's `gpu::k_*` kernels
+ * (include/cuda/kernels/0N_*.cuh) and CPU stages (include/kernels/0N_*.hpp);
+ * each declaration cites the interface it replaces.
+ */
+#ifndef OCTREE_PIPELINE_H
+#define OCTREE_PIPELINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ocp_context* ocp_ctx;
+
+typedef enum ocp_status {
+  OCP_OK = 0,
+  OCP_ERR_INVALID = 1,   /* bad argument: null pointer, size, table */
+  OCP_ERR_HIP = 2,       /* a HIP runtime call failed */
+  OCP_ERR_CAPACITY = 3,  /* an input exceeds a documented maximum */
+  OCP_ERR_STATE = 4      /* no result yet, or the call sequence is wrong */
+} ocp_status;
+
+/* include/shared/types.h:28-34 (60 bytes, 4-byte aligned). */
+typedef struct ocp_oct_node {
+  int32_t children[8];
+  float corner[4];
+  float cell_size;
+  int32_t child_node_mask;
+  int32_t child_leaf_mask;
+} ocp_oct_node;
+
+/* ------------------------------------------------------------ context */
+/* stream: a hipStream_t, or NULL for the context's own stream. */
+int ocp_create(ocp_ctx* out, int device, void* stream);
+int ocp_destroy(ocp_ctx ctx);
+const char* ocp_last_error(ocp_ctx ctx);
+int ocp_synchronize(ocp_ctx ctx);
+/* Plumbing for callers without their own allocator (tests, bindings). */
+int ocp_malloc(ocp_ctx ctx, void** ptr, size_t bytes);
+int ocp_free(ocp_ctx ctx, void* ptr);
+int ocp_memcpy_h2d(ocp_ctx ctx, void* dst, const void* src, size_t bytes);
+int ocp_memcpy_d2h(ocp_ctx ctx, void* dst, const void* src, size_t bytes);
+int ocp_memset(ocp_ctx ctx, void* dst, int value, size_t bytes);
+
+/* ------------------------------------------------------------ stage 1 */
+/* Replaces gpu::k_ComputeMortonCode (include/cuda/kernels/01_morton.cuh:7-11)
+ * and k_ComputeMortonCode (include/kernels/01_morton.hpp:9-13).
+ * data: n float4, 16-byte aligned. */
+int ocp_k_ComputeMortonCode(ocp_ctx ctx, const float* data,
+                            uint32_t* morton_keys, size_t n, float min_coord,
+                            float range);
+/* Same with a packed float3 input (12 bytes per point). */
+int ocp_k_ComputeMortonCode3(ocp_ctx ctx, const float* data,
+                             uint32_t* morton_keys, size_t n, float min_coord,
+                             float range);
+
+/* ------------------------------------------------------------ stage 2 */
+/* Replaces gpu::k_GlobalHistogram_WithLogicalBlocks
+ * (include/cuda/kernels/02_sort.cuh:21-24): raw 4 x 256 digit counts of
+ * `size` keys into global_histogram[1024] (zeroed by the call). */
+int ocp_k_GlobalHistogram(ocp_ctx ctx, const uint32_t* sort,
+                          uint32_t* global_histogram, uint32_t size);
+/* Replaces gpu::k_DigitBinning_WithLogicalBlocks
+ * (include/cuda/kernels/02_sort.cuh:25-33): one stable OneSweep pass on the
+ * digit at radix_shift (0, 8, 16, 24) from `sort` into `alt`, using the
+ * counts of ocp_k_GlobalHistogram. The lookback scratch and tile counter
+ * the reference takes as passHistogram/index live in the context and need
+ * no reset between calls. */
+int ocp_k_DigitBinning(ocp_ctx ctx, const uint32_t* global_histogram,
+                       const uint32_t* sort, uint32_t* alt, uint32_t size,
+                       uint32_t radix_shift);
+/* Replaces k_SimpleRadixSort(keys, keys_alt, n)
+ * (include/kernels/02_sort.hpp:7) and the reference's DispatchSortKernels
+ * (apps/gpu_demo/main.cu:63-125): histogram + 4 passes
+ * keys -> alt -> keys -> alt -> keys; the sorted keys end in `keys`. */
+int ocp_k_SortKeys(ocp_ctx ctx, uint32_t* keys, uint32_t* keys_alt,
+                   uint32_t n);
+
+/* ------------------------------------------------------------ stage 3 */
+/* Replaces gpu::k_CountUnique (include/cuda/kernels/03_unique.cuh:5) and
+ * k_CountUnique (include/kernels/03_unique.hpp:4): in-place unique of n
+ * sorted keys; the count is written to num_unique_out (device int32). */
+int ocp_k_CountUnique(ocp_ctx ctx, uint32_t* keys, int32_t* num_unique_out,
+                      int32_t n);
+
+/* ------------------------------------------------------------ stage 4 */
+/* Replaces gpu::k_BuildRadixTree (include/cuda/kernels/04_radix_tree.cuh:7-13)
+ * and k_BuildRadixTree (include/kernels/04_radix_tree.hpp:5-11): n nodes
+ * over n+1 sorted unique codes; pass n = U-1. parent[0] is written 0. */
+int ocp_k_BuildRadixTree(ocp_ctx ctx, int32_t n, const uint32_t* codes,
+                         uint8_t* prefix_n, uint8_t* has_leaf_left,
+                         uint8_t* has_leaf_right, int32_t* left_child,
+                         int32_t* parent);
+
+/* ------------------------------------------------------------ stage 5 */
+/* Replaces gpu::k_EdgeCount (include/cuda/kernels/05_edge_count.cuh:7-10)
+ * and k_EdgeCount (include/kernels/05_edge_count.hpp:5-8). */
+int ocp_k_EdgeCount(ocp_ctx ctx, const uint8_t* prefix_n,
+                    const int32_t* parents, int32_t* edge_count,
+                    int32_t n_brt_nodes);
+
+/* ------------------------------------------------------------ stage 6 */
+/* Replaces k_PartialSum (include/kernels/06_partial_sum.hpp:6-9), which the
+ * reference GPU path runs on the host (apps/gpu_demo/main.cu:308-310):
+ * inclusive scan of data[first, last) into d_first. */
+int ocp_k_PartialSum(ocp_ctx ctx, const int32_t* data, int64_t first,
+                     int64_t last, int32_t* d_first);
+
+/* ------------------------------------------------------------ stage 7 */
+/* The host-side root setup of apps/cpu_demo/main.cpp:120-132 on device:
+ * zero n_alloc nodes, then node 0 = corner of codes[0]'s root-level prefix
+ * (root level = prefix_n[0]/3, 0 when n_brt == 0), cell_size = range. */
+int ocp_k_InitOctNodes(ocp_ctx ctx, ocp_oct_node* nodes, int32_t n_alloc,
+                       const uint32_t* codes, const uint8_t* prefix_n,
+                       int32_t n_brt, float min_coord, float range);
+/* Replaces gpu::k_MakeOctNodes (include/cuda/kernels/07_octree.cuh:7-17) and
+ * k_MakeOctNodes (include/kernels/07_octree.hpp:5-14). N = number of BRT
+ * nodes; n_alloc = number of node slots (max(1, max_i P[i]+e[i])), which
+ * bounds every write. Expects nodes prepared by ocp_k_InitOctNodes. */
+int ocp_k_MakeOctNodes(ocp_ctx ctx, ocp_oct_node* oct_nodes,
+                       const int32_t* node_offsets,
+                       const int32_t* rt_node_counts, const uint32_t* codes,
+                       const uint8_t* rt_prefixN, const int32_t* rt_parents,
+                       float min_coord, float range, int32_t N,
+                       int32_t n_alloc);
+/* Replaces gpu::k_LinkLeafNodes (include/cuda/kernels/07_octree.cuh:19-28)
+ * and k_LinkLeafNodes (include/kernels/07_octree.hpp:16-25). */
+int ocp_k_LinkLeafNodes(ocp_ctx ctx, ocp_oct_node* nodes,
+                        const int32_t* node_offsets,
+                        const int32_t* rt_node_counts, const uint32_t* codes,
+                        const uint8_t* rt_hasLeafLeft,
+                        const uint8_t* rt_hasLeafRight,
+                        const uint8_t* rt_prefixN, const int32_t* rt_parents,
+                        const int32_t* rt_leftChild, int32_t N,
+                        int32_t n_alloc);
+
+/* ------------------------------------------------------------ pipeline */
+/* The whole chain of apps/gpu_demo/main.cu:157-356 / benchmarks/cuda/
+ * fixture.cuh:26-69 on device, with no host round trip: stage outputs stay
+ * in context-owned buffers (ocp_results) and the counts U and K are written
+ * to device memory. points: n float4 (float3 != 0: packed float3).
+ * Replays a captured hipGraph when graph mode is on. */
+int ocp_build(ocp_ctx ctx, const float* points, int float3, int64_t n,
+              float min_coord, float range);
+/* Batched form: n_seg independent clouds, cloud k = points
+ * [seg_offsets[k], seg_offsets[k+1]) with its own (seg_min[k],
+ * seg_range[k]); one segmented launch per stage. Host arrays. */
+int ocp_build_batched(ocp_ctx ctx, const float* points, int float3,
+                      const int32_t* seg_offsets, const float* seg_min,
+                      const float* seg_range, int32_t n_seg);
+/* Capture each distinct build shape once into a hipGraph and replay it. */
+int ocp_set_graph_mode(ocp_ctx ctx, int enable);
+
+/* Device pointers to the last build's outputs, indexed by point offset of
+ * each segment (node arrays by oct_start). */
+typedef struct ocp_results {
+  const uint32_t* sorted_keys;    /* n */
+  const uint32_t* unique_keys;    /* first U_k entries of each segment */
+  const uint8_t* prefix_n;        /* first U_k - 1 entries of each segment */
+  const uint8_t* has_leaf_left;
+  const uint8_t* has_leaf_right;
+  const int32_t* left_child;
+  const int32_t* parent;
+  const int32_t* edge_count;
+  const int32_t* offsets;         /* inclusive scan of edge_count */
+  const ocp_oct_node* nodes;      /* segment k at oct_start[k] */
+  const int32_t* n_unique;        /* device, n_seg */
+  const int32_t* n_oct;           /* device, n_seg: K_k */
+  const int32_t* oct_start;       /* device, n_seg + 1 */
+  int32_t n_seg;
+  int64_t n_points;
+} ocp_results;
+int ocp_get_results(ocp_ctx ctx, ocp_results* out);
+/* Synchronises and copies the per-segment U and K (host arrays of n_seg). */
+int ocp_get_counts(ocp_ctx ctx, int32_t* n_unique, int32_t* n_oct);
+
+/* Per-stage device time of the last eager build, in ms, measured with HIP
+ * events on the context stream when stage timing is on: [morton, sort,
+ * unique, radix_tree, edge_scan, octree, total]. */
+int ocp_set_stage_timing(ocp_ctx ctx, int enable);
+int ocp_get_stage_ms(ocp_ctx ctx, float* ms7);
+/* Time `steps` builds (after `warmup`) of the configured kind on the
+ * context stream with HIP events, bracketed by device synchronisation.
+ * Writes the mean ms per build. */
+int ocp_time_builds(ocp_ctx ctx, const float* points, int float3,
+                    const int32_t* seg_offsets, const float* seg_min,
+                    const float* seg_range, int32_t n_seg, int warmup,
+                    int steps, float* ms_per_build);
+
+/* ------------------------------------------------------------ tuning */
+/* Tile parameters: sort_waves, sort_kpt, morton_ppt, morton_grid, scan_ipt. */
+typedef struct ocp_tune {
+  int32_t sort_waves;
+  int32_t sort_kpt;
+  int32_t morton_ppt;
+  int32_t morton_grid;
+  int32_t scan_ipt;
+} ocp_tune;
+int ocp_get_tune(ocp_ctx ctx, ocp_tune* out);
+int ocp_set_tune(ocp_ctx ctx, const ocp_tune* in);
+/* Time every supported configuration on the given workload (plain when
+ * n_seg == 0, else batched) and keep the fastest; writes the choice. */
+int ocp_autotune(ocp_ctx ctx, const float* points, int float3, int64_t n,
+                 const int32_t* seg_offsets, const float* seg_min,
+                 const float* seg_range, int32_t n_seg, int reps,
+                 ocp_tune* chosen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OCTREE_PIPELINE_H */

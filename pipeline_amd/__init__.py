@@ -1,0 +1,438 @@
+"""MI355X (gfx950) octree-construction pipeline - Python host binding.
+
+A thin ctypes layer over the C ABI in ``include/octree_pipeline.h``
+(``pipeline_amd/_lib/libocpipe.so``). It mirrors the reference's per-stage
+entry points (``k_ComputeMortonCode``, ``k_SortKeys``/``k_SimpleRadixSort``,
+``k_CountUnique``, ``k_BuildRadixTree``, ``k_EdgeCount``, ``k_PartialSum``,
+``k_MakeOctNodes``, ``k_LinkLeafNodes``; include/kernels/all.hpp and
+include/cuda/kernels/all.cuh of ucsc-redwood/pipeline) with the same
+argument meaning, and exposes the fused device-resident build.
+
+There is no CPU fallback: importing works without a GPU, but creating a
+``Context`` loads the HIP library and fails loudly when it is missing or no
+device is present.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libocpipe.so")
+
+OCT_NODE_DTYPE = np.dtype(
+    [
+        ("children", "<i4", (8,)),
+        ("corner", "<f4", (4,)),
+        ("cell_size", "<f4"),
+        ("child_node_mask", "<i4"),
+        ("child_leaf_mask", "<i4"),
+    ]
+)
+assert OCT_NODE_DTYPE.itemsize == 60
+
+OCP_OK = 0
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_U32 = ctypes.c_uint32
+_F = ctypes.c_float
+_INT = ctypes.c_int
+_SZ = ctypes.c_size_t
+
+
+class OcpError(RuntimeError):
+    pass
+
+
+class Results(ctypes.Structure):
+    _fields_ = [
+        ("sorted_keys", _P), ("unique_keys", _P), ("prefix_n", _P),
+        ("has_leaf_left", _P), ("has_leaf_right", _P), ("left_child", _P),
+        ("parent", _P), ("edge_count", _P), ("offsets", _P), ("nodes", _P),
+        ("n_unique", _P), ("n_oct", _P), ("oct_start", _P),
+        ("n_seg", _I32), ("n_points", _I64),
+    ]
+
+
+class Tune(ctypes.Structure):
+    _fields_ = [("sort_waves", _I32), ("sort_kpt", _I32), ("morton_ppt", _I32),
+                ("morton_grid", _I32), ("scan_ipt", _I32)]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
+# Every function the C ABI exports, with its ctypes signature. The CPU test
+# suite checks that the built library exports exactly these.
+SIGNATURES = {
+    "ocp_create": (_INT, [ctypes.POINTER(_P), _INT, _P]),
+    "ocp_destroy": (_INT, [_P]),
+    "ocp_last_error": (ctypes.c_char_p, [_P]),
+    "ocp_synchronize": (_INT, [_P]),
+    "ocp_malloc": (_INT, [_P, ctypes.POINTER(_P), _SZ]),
+    "ocp_free": (_INT, [_P, _P]),
+    "ocp_memcpy_h2d": (_INT, [_P, _P, _P, _SZ]),
+    "ocp_memcpy_d2h": (_INT, [_P, _P, _P, _SZ]),
+    "ocp_memset": (_INT, [_P, _P, _INT, _SZ]),
+    "ocp_k_ComputeMortonCode": (_INT, [_P, _P, _P, _SZ, _F, _F]),
+    "ocp_k_ComputeMortonCode3": (_INT, [_P, _P, _P, _SZ, _F, _F]),
+    "ocp_k_GlobalHistogram": (_INT, [_P, _P, _P, _U32]),
+    "ocp_k_DigitBinning": (_INT, [_P, _P, _P, _P, _U32, _U32]),
+    "ocp_k_SortKeys": (_INT, [_P, _P, _P, _U32]),
+    "ocp_k_CountUnique": (_INT, [_P, _P, _P, _I32]),
+    "ocp_k_BuildRadixTree": (_INT, [_P, _I32, _P, _P, _P, _P, _P, _P]),
+    "ocp_k_EdgeCount": (_INT, [_P, _P, _P, _P, _I32]),
+    "ocp_k_PartialSum": (_INT, [_P, _P, _I64, _I64, _P]),
+    "ocp_k_InitOctNodes": (_INT, [_P, _P, _I32, _P, _P, _I32, _F, _F]),
+    "ocp_k_MakeOctNodes": (_INT, [_P, _P, _P, _P, _P, _P, _P, _F, _F, _I32, _I32]),
+    "ocp_k_LinkLeafNodes": (_INT, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32]),
+    "ocp_build": (_INT, [_P, _P, _INT, _I64, _F, _F]),
+    "ocp_build_batched": (_INT, [_P, _P, _INT, _P, _P, _P, _I32]),
+    "ocp_set_graph_mode": (_INT, [_P, _INT]),
+    "ocp_get_results": (_INT, [_P, ctypes.POINTER(Results)]),
+    "ocp_get_counts": (_INT, [_P, _P, _P]),
+    "ocp_set_stage_timing": (_INT, [_P, _INT]),
+    "ocp_get_stage_ms": (_INT, [_P, _P]),
+    "ocp_time_builds": (_INT, [_P, _P, _INT, _P, _P, _P, _I32, _INT, _INT, ctypes.POINTER(_F)]),
+    "ocp_get_tune": (_INT, [_P, ctypes.POINTER(Tune)]),
+    "ocp_set_tune": (_INT, [_P, ctypes.POINTER(Tune)]),
+    "ocp_autotune": (_INT, [_P, _P, _INT, _I64, _P, _P, _P, _I32, _INT, ctypes.POINTER(Tune)]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libocpipe.so (raises OcpError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OcpError(
+            f"HIP library {path} is missing: build it with "
+            "`python -m pipeline_amd.build` (there is no CPU fallback)")
+    L = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class DeviceArray:
+    """A device allocation with a numpy dtype/shape, owned by a Context."""
+
+    def __init__(self, ctx: "Context", shape, dtype):
+        self.ctx = ctx
+        self.dtype = np.dtype(dtype)
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        self.ptr = ctx._malloc(self.nbytes)
+
+    @property
+    def size(self) -> int:
+        return int(np.prod(self.shape, dtype=np.int64))
+
+    def copy_from(self, host: np.ndarray) -> "DeviceArray":
+        h = np.ascontiguousarray(host, dtype=self.dtype)
+        if h.nbytes > self.nbytes:
+            raise OcpError("host array larger than the device buffer")
+        self.ctx._check(self.ctx.lib.ocp_memcpy_h2d(self.ctx.h, self.ptr, _ptr(h), h.nbytes))
+        return self
+
+    def to_host(self, count: int | None = None) -> np.ndarray:
+        n = self.size if count is None else int(count)
+        out = np.empty(n, self.dtype)
+        if n:
+            self.ctx._check(self.ctx.lib.ocp_memcpy_d2h(self.ctx.h, _ptr(out), self.ptr,
+                                                        n * self.dtype.itemsize))
+        return out.reshape(self.shape) if count is None else out
+
+    def zero(self) -> "DeviceArray":
+        self.ctx._check(self.ctx.lib.ocp_memset(self.ctx.h, self.ptr, 0, self.nbytes))
+        return self
+
+    def free(self):
+        if self.ptr:
+            self.ctx.lib.ocp_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+
+def d2h(ctx: "Context", ptr: int, count: int, dtype) -> np.ndarray:
+    """Copy `count` elements of `dtype` from a raw device pointer."""
+    dt = np.dtype(dtype)
+    out = np.empty(int(count), dt)
+    if count:
+        ctx._check(ctx.lib.ocp_memcpy_d2h(ctx.h, _ptr(out), ctypes.c_void_p(ptr),
+                                          out.nbytes))
+    return out
+
+
+@dataclass
+class BuildOutput:
+    """Host copies of one build's stage outputs, per segment."""
+    sorted_keys: list
+    unique_keys: list
+    prefix_n: list
+    has_leaf_left: list
+    has_leaf_right: list
+    left_child: list
+    parent: list
+    edge_count: list
+    offsets: list
+    nodes: list
+    n_unique: np.ndarray
+    n_oct: np.ndarray
+
+
+class Context:
+    """One HIP stream + workspace (ocp_ctx). Not thread safe."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self.lib.ocp_create(ctypes.byref(h), device, None)
+        if rc != OCP_OK:
+            raise OcpError(f"ocp_create failed with status {rc} (no usable GPU?)")
+        self.h = h
+        self._owned: list[DeviceArray] = []
+
+    # ------------------------------------------------------------ plumbing
+    def _check(self, rc: int):
+        if rc != OCP_OK:
+            msg = self.lib.ocp_last_error(self.h)
+            raise OcpError(f"status {rc}: {msg.decode() if msg else ''}")
+
+    def _malloc(self, nbytes: int) -> int:
+        p = ctypes.c_void_p()
+        self._check(self.lib.ocp_malloc(self.h, ctypes.byref(p), max(int(nbytes), 1)))
+        return p.value
+
+    def empty(self, shape, dtype) -> DeviceArray:
+        a = DeviceArray(self, shape, dtype)
+        self._owned.append(a)
+        return a
+
+    def zeros(self, shape, dtype) -> DeviceArray:
+        return self.empty(shape, dtype).zero()
+
+    def upload(self, host: np.ndarray, dtype=None) -> DeviceArray:
+        h = np.ascontiguousarray(host, dtype=dtype or host.dtype)
+        return self.empty(h.shape, h.dtype).copy_from(h)
+
+    def synchronize(self):
+        self._check(self.lib.ocp_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            for a in self._owned:
+                a.free()
+            self._owned.clear()
+            self.lib.ocp_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ stages
+    def k_ComputeMortonCode(self, data: DeviceArray, keys: DeviceArray, n: int,
+                            min_coord: float, range_: float):
+        fn = (self.lib.ocp_k_ComputeMortonCode3 if data.shape[-1] == 3
+              else self.lib.ocp_k_ComputeMortonCode)
+        self._check(fn(self.h, data.ptr, keys.ptr, n, min_coord, range_))
+
+    def k_GlobalHistogram(self, keys: DeviceArray, hist: DeviceArray, n: int):
+        self._check(self.lib.ocp_k_GlobalHistogram(self.h, keys.ptr, hist.ptr, n))
+
+    def k_DigitBinning(self, hist, keys, alt, n: int, radix_shift: int):
+        self._check(self.lib.ocp_k_DigitBinning(self.h, hist.ptr, keys.ptr, alt.ptr, n,
+                                                radix_shift))
+
+    def k_SortKeys(self, keys: DeviceArray, keys_alt: DeviceArray, n: int):
+        self._check(self.lib.ocp_k_SortKeys(self.h, keys.ptr, keys_alt.ptr, n))
+
+    k_SimpleRadixSort = k_SortKeys
+
+    def k_CountUnique(self, keys: DeviceArray, n: int) -> int:
+        out = self.empty(1, np.int32)
+        self._check(self.lib.ocp_k_CountUnique(self.h, keys.ptr, out.ptr, n))
+        u = int(out.to_host()[0])
+        out.free()
+        self._owned.remove(out)
+        return u
+
+    def k_BuildRadixTree(self, n: int, codes, prefix_n, has_leaf_left,
+                         has_leaf_right, left_child, parent):
+        self._check(self.lib.ocp_k_BuildRadixTree(
+            self.h, n, codes.ptr, prefix_n.ptr, has_leaf_left.ptr, has_leaf_right.ptr,
+            left_child.ptr, parent.ptr))
+
+    def k_EdgeCount(self, prefix_n, parents, edge_count, n_brt_nodes: int):
+        self._check(self.lib.ocp_k_EdgeCount(self.h, prefix_n.ptr, parents.ptr,
+                                             edge_count.ptr, n_brt_nodes))
+
+    def k_PartialSum(self, data, first: int, last: int, d_first):
+        self._check(self.lib.ocp_k_PartialSum(self.h, data.ptr, first, last, d_first.ptr))
+
+    def k_InitOctNodes(self, nodes, n_alloc, codes, prefix_n, n_brt, min_coord, range_):
+        self._check(self.lib.ocp_k_InitOctNodes(
+            self.h, nodes.ptr, n_alloc, codes.ptr, prefix_n.ptr if prefix_n else None,
+            n_brt, min_coord, range_))
+
+    def k_MakeOctNodes(self, nodes, node_offsets, rt_node_counts, codes, rt_prefixN,
+                       rt_parents, min_coord, range_, N, n_alloc):
+        self._check(self.lib.ocp_k_MakeOctNodes(
+            self.h, nodes.ptr, node_offsets.ptr, rt_node_counts.ptr, codes.ptr,
+            rt_prefixN.ptr, rt_parents.ptr, min_coord, range_, N, n_alloc))
+
+    def k_LinkLeafNodes(self, nodes, node_offsets, rt_node_counts, codes,
+                        rt_hasLeafLeft, rt_hasLeafRight, rt_prefixN, rt_parents,
+                        rt_leftChild, N, n_alloc):
+        self._check(self.lib.ocp_k_LinkLeafNodes(
+            self.h, nodes.ptr, node_offsets.ptr, rt_node_counts.ptr, codes.ptr,
+            rt_hasLeafLeft.ptr, rt_hasLeafRight.ptr, rt_prefixN.ptr, rt_parents.ptr,
+            rt_leftChild.ptr, N, n_alloc))
+
+    # ------------------------------------------------------------ pipeline
+    def set_graph_mode(self, enable: bool):
+        self._check(self.lib.ocp_set_graph_mode(self.h, int(bool(enable))))
+
+    def set_stage_timing(self, enable: bool):
+        self._check(self.lib.ocp_set_stage_timing(self.h, int(bool(enable))))
+
+    def stage_ms(self) -> dict:
+        ms = np.zeros(7, np.float32)
+        self._check(self.lib.ocp_get_stage_ms(self.h, _ptr(ms)))
+        names = ["morton", "sort", "unique", "radix_tree", "edge_scan", "octree", "total"]
+        return dict(zip(names, ms.astype(float).tolist()))
+
+    def build(self, points: DeviceArray, n: int | None = None, min_coord=0.0,
+              range_=1024.0):
+        n = points.shape[0] if n is None else n
+        f3 = int(points.shape[-1] == 3)
+        self._check(self.lib.ocp_build(self.h, points.ptr, f3, n, min_coord, range_))
+
+    def build_batched(self, points: DeviceArray, seg_offsets, seg_min, seg_range):
+        offs = np.ascontiguousarray(seg_offsets, np.int32)
+        mn = np.ascontiguousarray(seg_min, np.float32)
+        rg = np.ascontiguousarray(seg_range, np.float32)
+        f3 = int(points.shape[-1] == 3)
+        self._check(self.lib.ocp_build_batched(self.h, points.ptr, f3, _ptr(offs),
+                                               _ptr(mn), _ptr(rg), offs.size - 1))
+
+    def results(self) -> Results:
+        r = Results()
+        self._check(self.lib.ocp_get_results(self.h, ctypes.byref(r)))
+        return r
+
+    def counts(self, n_seg: int):
+        u = np.zeros(n_seg, np.int32)
+        k = np.zeros(n_seg, np.int32)
+        self._check(self.lib.ocp_get_counts(self.h, _ptr(u), _ptr(k)))
+        return u, k
+
+    def fetch(self, seg_offsets=None) -> BuildOutput:
+        """Copy every stage output of the last build to the host."""
+        r = self.results()
+        n_seg = r.n_seg
+        offs = (np.array([0, r.n_points], np.int64) if seg_offsets is None
+                else np.asarray(seg_offsets, np.int64))
+        u, k = self.counts(n_seg)
+        total = int(r.n_points)
+        sk = d2h(self, r.sorted_keys, total, np.uint32)
+        uk = d2h(self, r.unique_keys, total, np.uint32)
+        pn = d2h(self, r.prefix_n, total, np.uint8)
+        hl = d2h(self, r.has_leaf_left, total, np.uint8)
+        hr = d2h(self, r.has_leaf_right, total, np.uint8)
+        lc = d2h(self, r.left_child, total, np.int32)
+        par = d2h(self, r.parent, total, np.int32)
+        e = d2h(self, r.edge_count, total, np.int32)
+        p = d2h(self, r.offsets, total, np.int32)
+        ost = d2h(self, r.oct_start, n_seg + 1, np.int32)
+        nodes = d2h(self, r.nodes, int(ost[-1]) * 60, np.uint8).view(OCT_NODE_DTYPE)
+        out = BuildOutput([], [], [], [], [], [], [], [], [], [], u, k)
+        for s in range(n_seg):
+            a, b = int(offs[s]), int(offs[s + 1])
+            m = max(int(u[s]) - 1, 0)
+            out.sorted_keys.append(sk[a:b])
+            out.unique_keys.append(uk[a:a + int(u[s])])
+            out.prefix_n.append(pn[a:a + m])
+            out.has_leaf_left.append(hl[a:a + m])
+            out.has_leaf_right.append(hr[a:a + m])
+            out.left_child.append(lc[a:a + m])
+            out.parent.append(par[a:a + m])
+            out.edge_count.append(e[a:a + m])
+            out.offsets.append(p[a:a + m])
+            out.nodes.append(nodes[int(ost[s]):int(ost[s]) + int(k[s])])
+        return out
+
+    def time_builds(self, points: DeviceArray, warmup: int, steps: int,
+                    seg_offsets=None, seg_min=None, seg_range=None, n=None,
+                    min_coord=0.0, range_=1024.0) -> float:
+        f3 = int(points.shape[-1] == 3)
+        ms = ctypes.c_float()
+        if seg_offsets is None:
+            n = points.shape[0] if n is None else n
+            offs = np.array([0, n], np.int32)
+            mn = np.array([min_coord], np.float32)
+            rg = np.array([range_], np.float32)
+            n_seg = 0
+        else:
+            offs = np.ascontiguousarray(seg_offsets, np.int32)
+            mn = np.ascontiguousarray(seg_min, np.float32)
+            rg = np.ascontiguousarray(seg_range, np.float32)
+            n_seg = offs.size - 1
+        self._check(self.lib.ocp_time_builds(self.h, points.ptr, f3, _ptr(offs), _ptr(mn),
+                                             _ptr(rg), n_seg, warmup, steps,
+                                             ctypes.byref(ms)))
+        return float(ms.value)
+
+    def get_tune(self) -> dict:
+        t = Tune()
+        self._check(self.lib.ocp_get_tune(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tune(self, **kw):
+        t = Tune()
+        self._check(self.lib.ocp_get_tune(self.h, ctypes.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        self._check(self.lib.ocp_set_tune(self.h, ctypes.byref(t)))
+
+    def autotune(self, points: DeviceArray, n=None, seg_offsets=None, seg_min=None,
+                 seg_range=None, reps=3) -> dict:
+        t = Tune()
+        f3 = int(points.shape[-1] == 3)
+        if seg_offsets is None:
+            n = points.shape[0] if n is None else n
+            mn = np.array([0.0], np.float32)
+            rg = np.array([1024.0], np.float32)
+            self._check(self.lib.ocp_autotune(self.h, points.ptr, f3, n, None, _ptr(mn),
+                                              _ptr(rg), 0, reps, ctypes.byref(t)))
+        else:
+            offs = np.ascontiguousarray(seg_offsets, np.int32)
+            mn = np.ascontiguousarray(seg_min, np.float32)
+            rg = np.ascontiguousarray(seg_range, np.float32)
+            self._check(self.lib.ocp_autotune(self.h, points.ptr, f3, int(offs[-1]),
+                                              _ptr(offs), _ptr(mn), _ptr(rg), offs.size - 1,
+                                              reps, ctypes.byref(t)))
+        return t.as_dict()

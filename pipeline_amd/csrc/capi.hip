@@ -1,0 +1,975 @@
+// C ABI of include/octree_pipeline.h: context, device-memory plumbing, the
+// per-stage entry points, the fused device-resident pipeline (plain and
+// batched), hipGraph capture/replay, stage timing and the autotuner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/octree_pipeline.h"
+#include "ocp_internal.hpp"
+
+using namespace ocp;
+
+static_assert(sizeof(ocp_oct_node) == sizeof(OctNode), "node layout");
+
+namespace {
+
+constexpr int64_t kMaxPoints = (int64_t(1) << 30) - 1;  // 30-bit flag counts
+constexpr int64_t kMaxSegPoints = int64_t(1) << 27;     // octree order keys
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  int64_t n = 0;
+};
+
+struct Workspace {
+  int64_t n_cap = 0;
+  int32_t seg_cap = 0;
+  int64_t k_cap = 0;
+  int64_t flag_cap = 0;
+  uint32_t* keys_a = nullptr;
+  uint32_t* keys_b = nullptr;
+  uint8_t* prefix_n = nullptr;
+  uint8_t* hl = nullptr;
+  uint8_t* hr = nullptr;
+  int32_t* lc = nullptr;
+  int32_t* par = nullptr;
+  int32_t* edge = nullptr;
+  int32_t* offs = nullptr;
+  OctNode* nodes = nullptr;
+  uint32_t* hist = nullptr;      // seg_cap * 1024
+  uint64_t* flags = nullptr;     // flag_cap words
+  int32_t* counts = nullptr;     // n_unique[seg_cap], kalloc[seg_cap]
+  int32_t* oct_start = nullptr;  // seg_cap + 1
+  int32_t* tables = nullptr;     // seg_start + 4 tile maps, 5 * (seg_cap+1)
+  float* boxes = nullptr;        // seg_min[seg_cap], seg_range[seg_cap]
+};
+
+struct Plan {
+  int32_t n_seg = 1;
+  int64_t n_total = 0;
+  SegView sv{};
+  int32_t tiles_morton = 0, tiles_sort = 0, tiles_scan = 0, tiles_node = 0;
+  const int32_t* ts_morton = nullptr;
+  const int32_t* ts_sort = nullptr;
+  const int32_t* ts_scan = nullptr;
+  const int32_t* ts_node = nullptr;
+};
+
+struct LastBuild {
+  bool valid = false;
+  bool batched = false;
+  const float* pts = nullptr;
+  int float3 = 0;
+  int64_t n = 0;
+  float mn = 0, rg = 0;
+  std::vector<int32_t> offsets;
+  std::vector<float> seg_min, seg_range;
+};
+
+}  // namespace
+
+struct ocp_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  TuneTable tune;
+  std::string err;
+  uint32_t* ctrl = nullptr;
+  // per-stage scratch
+  uint64_t* s_flags = nullptr;
+  int64_t s_flag_cap = 0;
+  uint32_t* s_hist = nullptr;
+  Workspace ws;
+  // batch tables currently on device
+  std::vector<int32_t> tab_offsets;
+  std::vector<float> tab_min, tab_rg;
+  int tab_tune_key = -1;
+  bool have_result = false;
+  Plan last_plan;
+  bool graph_mode = false;
+  std::map<std::string, hipGraphExec_t> graphs;
+  bool stage_timing = false;
+  hipEvent_t ev[8] = {};
+  bool ev_recorded = false;
+  LastBuild last;
+};
+
+namespace {
+
+int fail(ocp_ctx c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define OCP_HIP(c, expr)                                                   \
+  do {                                                                     \
+    hipError_t e_ = (expr);                                                \
+    if (e_ != hipSuccess)                                                  \
+      return fail(c, OCP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define OCP_CHECK_LAUNCH(c) OCP_HIP(c, hipGetLastError())
+
+template <typename T>
+int dev_alloc(ocp_ctx c, T** p, int64_t n) {
+  *p = nullptr;
+  if (n <= 0) n = 1;
+  OCP_HIP(c, hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * static_cast<size_t>(n)));
+  return OCP_OK;
+}
+template <typename T>
+void dev_free(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+void clear_graphs(ocp_ctx c) {
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+}
+
+int ensure_scratch(ocp_ctx c, int64_t flag_words) {
+  if (flag_words > c->s_flag_cap) {
+    dev_free(c->s_flags);
+    const int64_t cap = std::max<int64_t>(flag_words, 1 << 16);
+    if (int r = dev_alloc(c, &c->s_flags, cap)) return r;
+    OCP_HIP(c, hipMemsetAsync(c->s_flags, 0, sizeof(uint64_t) * cap, c->stream));
+    c->s_flag_cap = cap;
+  }
+  return OCP_OK;
+}
+
+void free_workspace(Workspace& w) {
+  dev_free(w.keys_a); dev_free(w.keys_b); dev_free(w.prefix_n); dev_free(w.hl);
+  dev_free(w.hr); dev_free(w.lc); dev_free(w.par); dev_free(w.edge);
+  dev_free(w.offs); dev_free(w.nodes); dev_free(w.hist); dev_free(w.flags);
+  dev_free(w.counts); dev_free(w.oct_start); dev_free(w.tables); dev_free(w.boxes);
+  w = Workspace{};
+}
+
+int alloc_nodes(ocp_ctx c, int64_t k_cap) {
+  Workspace& w = c->ws;
+  dev_free(w.nodes);
+  // 16 bytes of slack: the zero-fill kernel rounds up to 16-byte stores.
+  const int64_t bytes = k_cap * static_cast<int64_t>(sizeof(OctNode)) + 16;
+  OCP_HIP(c, hipMalloc(reinterpret_cast<void**>(&w.nodes), static_cast<size_t>(bytes)));
+  w.k_cap = k_cap;
+  return OCP_OK;
+}
+
+int ensure_workspace(ocp_ctx c, int64_t n, int32_t n_seg, int64_t flag_words) {
+  Workspace& w = c->ws;
+  const bool grow_n = n > w.n_cap;
+  const bool grow_seg = n_seg > w.seg_cap;
+  const bool grow_flags = flag_words > w.flag_cap;
+  if (!grow_n && !grow_seg && !grow_flags) return OCP_OK;
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  clear_graphs(c);
+  c->have_result = false;
+  c->tab_offsets.clear();
+  if (grow_n) {
+    const int64_t cap = std::max<int64_t>(n, 4096);
+    dev_free(w.keys_a); dev_free(w.keys_b); dev_free(w.prefix_n); dev_free(w.hl);
+    dev_free(w.hr); dev_free(w.lc); dev_free(w.par); dev_free(w.edge); dev_free(w.offs);
+    int r = 0;
+    if ((r = dev_alloc(c, &w.keys_a, cap)) || (r = dev_alloc(c, &w.keys_b, cap)) ||
+        (r = dev_alloc(c, &w.prefix_n, cap)) || (r = dev_alloc(c, &w.hl, cap)) ||
+        (r = dev_alloc(c, &w.hr, cap)) || (r = dev_alloc(c, &w.lc, cap)) ||
+        (r = dev_alloc(c, &w.par, cap)) || (r = dev_alloc(c, &w.edge, cap)) ||
+        (r = dev_alloc(c, &w.offs, cap)))
+      return r;
+    w.n_cap = cap;
+    if (w.k_cap < cap)
+      if (int r2 = alloc_nodes(c, cap)) return r2;
+  }
+  if (grow_seg) {
+    const int32_t cap = std::max<int32_t>(n_seg, 1);
+    dev_free(w.hist); dev_free(w.counts); dev_free(w.oct_start); dev_free(w.tables);
+    dev_free(w.boxes);
+    int r = 0;
+    if ((r = dev_alloc(c, &w.hist, int64_t(cap) * 1024)) ||
+        (r = dev_alloc(c, &w.counts, int64_t(cap) * 2)) ||
+        (r = dev_alloc(c, &w.oct_start, int64_t(cap) + 1)) ||
+        (r = dev_alloc(c, &w.tables, 5 * (int64_t(cap) + 1))) ||
+        (r = dev_alloc(c, &w.boxes, 2 * int64_t(cap))))
+      return r;
+    w.seg_cap = cap;
+  }
+  if (grow_flags) {
+    dev_free(w.flags);
+    const int64_t cap = std::max<int64_t>(flag_words, 1 << 16);
+    if (int r = dev_alloc(c, &w.flags, cap)) return r;
+    OCP_HIP(c, hipMemsetAsync(w.flags, 0, sizeof(uint64_t) * cap, c->stream));
+    w.flag_cap = cap;
+  }
+  return OCP_OK;
+}
+
+SegView single_view(int64_t n, float mn, float rg) {
+  SegView sv{};
+  sv.n_seg = 1;
+  sv.start = nullptr;
+  sv.mn = nullptr;
+  sv.rg = nullptr;
+  sv.n1 = static_cast<int32_t>(n);
+  sv.min1 = mn;
+  sv.range1 = rg;
+  return sv;
+}
+
+int tune_key(const TuneTable& t) {
+  return ((t.sort_waves * 64 + t.sort_kpt) * 64 + t.morton_ppt) * 64 + t.scan_ipt;
+}
+
+int64_t flag_words_for(const TuneTable& tt, int64_t n, int32_t n_seg) {
+  const int64_t sort_tiles = cdiv(n, sort_tile(tt)) + n_seg;
+  const int64_t scan_tiles = cdiv(n, scan_tile(tt)) + n_seg;
+  return std::max(sort_tiles * 256, scan_tiles);
+}
+
+// Plain plan: one segment, scalar box, no device tables.
+Plan plain_plan(const TuneTable& tt, int64_t n, float mn, float rg) {
+  Plan p;
+  p.n_seg = 1;
+  p.n_total = n;
+  p.sv = single_view(n, mn, rg);
+  p.tiles_morton = static_cast<int32_t>(cdiv(n, morton_tile(tt)));
+  p.tiles_sort = static_cast<int32_t>(cdiv(n, sort_tile(tt)));
+  p.tiles_scan = static_cast<int32_t>(cdiv(n, scan_tile(tt)));
+  p.tiles_node = static_cast<int32_t>(cdiv(n, kNodeTile));
+  return p;
+}
+
+// Batched plan: validates the table, uploads offsets, boxes and tile maps
+// (only when they changed).
+int batched_plan(ocp_ctx c, const int32_t* offsets, const float* smin,
+                 const float* srg, int32_t n_seg, Plan* out) {
+  if (n_seg < 1) return fail(c, OCP_ERR_INVALID, "n_seg must be >= 1");
+  if (!offsets || !smin || !srg) return fail(c, OCP_ERR_INVALID, "null segment table");
+  if (offsets[0] != 0) return fail(c, OCP_ERR_INVALID, "seg_offsets[0] must be 0");
+  for (int32_t k = 0; k < n_seg; ++k) {
+    const int64_t len = int64_t(offsets[k + 1]) - offsets[k];
+    if (len < 0) return fail(c, OCP_ERR_INVALID, "seg_offsets must be non-decreasing");
+    if (len > kMaxSegPoints) return fail(c, OCP_ERR_CAPACITY, "segment larger than 2^27 points");
+    if (!(srg[k] > 0.0f)) return fail(c, OCP_ERR_INVALID, "seg_range must be > 0");
+  }
+  const int64_t n = offsets[n_seg];
+  if (n > kMaxPoints) return fail(c, OCP_ERR_CAPACITY, "more than 2^30-1 points");
+  const TuneTable& tt = c->tune;
+  if (int r = ensure_workspace(c, n, n_seg, flag_words_for(tt, n, n_seg))) return r;
+  Workspace& w = c->ws;
+  const int32_t stride = w.seg_cap + 1;
+  int32_t* d_start = w.tables;
+  int32_t* d_tm = w.tables + stride;
+  int32_t* d_ts = w.tables + 2 * stride;
+  int32_t* d_tc = w.tables + 3 * stride;
+  int32_t* d_tn = w.tables + 4 * stride;
+  Plan p;
+  p.n_seg = n_seg;
+  p.n_total = n;
+  std::vector<int32_t> host(5 * static_cast<size_t>(n_seg + 1));
+  int64_t tm = 0, ts = 0, tc = 0, tn = 0;
+  for (int32_t k = 0; k <= n_seg; ++k) {
+    host[k] = offsets[k];
+    host[(n_seg + 1) + k] = static_cast<int32_t>(tm);
+    host[2 * (n_seg + 1) + k] = static_cast<int32_t>(ts);
+    host[3 * (n_seg + 1) + k] = static_cast<int32_t>(tc);
+    host[4 * (n_seg + 1) + k] = static_cast<int32_t>(tn);
+    if (k < n_seg) {
+      const int64_t len = int64_t(offsets[k + 1]) - offsets[k];
+      tm += cdiv(len, morton_tile(tt));
+      ts += cdiv(len, sort_tile(tt));
+      tc += cdiv(len, scan_tile(tt));
+      tn += cdiv(len, kNodeTile);
+    }
+  }
+  const bool same = c->tab_tune_key == tune_key(tt) &&
+                    c->tab_offsets.size() == static_cast<size_t>(n_seg + 1) &&
+                    std::equal(offsets, offsets + n_seg + 1, c->tab_offsets.begin());
+  if (!same) {
+    for (int q = 0; q < 5; ++q)
+      OCP_HIP(c, hipMemcpyAsync(w.tables + q * stride, host.data() + q * (n_seg + 1),
+                                sizeof(int32_t) * (n_seg + 1), hipMemcpyHostToDevice,
+                                c->stream));
+    c->tab_offsets.assign(offsets, offsets + n_seg + 1);
+    c->tab_tune_key = tune_key(tt);
+    c->tab_min.clear();
+  }
+  const bool same_box = c->tab_min.size() == static_cast<size_t>(n_seg) &&
+                        std::equal(smin, smin + n_seg, c->tab_min.begin()) &&
+                        std::equal(srg, srg + n_seg, c->tab_rg.begin());
+  if (!same_box) {
+    OCP_HIP(c, hipMemcpyAsync(w.boxes, smin, sizeof(float) * n_seg, hipMemcpyHostToDevice, c->stream));
+    OCP_HIP(c, hipMemcpyAsync(w.boxes + w.seg_cap, srg, sizeof(float) * n_seg,
+                              hipMemcpyHostToDevice, c->stream));
+    OCP_HIP(c, hipStreamSynchronize(c->stream));  // host arrays may go away
+    c->tab_min.assign(smin, smin + n_seg);
+    c->tab_rg.assign(srg, srg + n_seg);
+  }
+  p.sv.n_seg = n_seg;
+  p.sv.start = d_start;
+  p.sv.mn = w.boxes;
+  p.sv.rg = w.boxes + w.seg_cap;
+  p.sv.n1 = static_cast<int32_t>(n);
+  p.sv.min1 = 0.0f;
+  p.sv.range1 = 1.0f;
+  p.tiles_morton = static_cast<int32_t>(tm);
+  p.tiles_sort = static_cast<int32_t>(ts);
+  p.tiles_scan = static_cast<int32_t>(tc);
+  p.tiles_node = static_cast<int32_t>(tn);
+  p.ts_morton = d_tm;
+  p.ts_sort = d_ts;
+  p.ts_scan = d_tc;
+  p.ts_node = d_tn;
+  *out = p;
+  return OCP_OK;
+}
+
+OctArgs pipeline_oct_args(ocp_ctx c) {
+  Workspace& w = c->ws;
+  OctArgs a{};
+  a.nodes = w.nodes;
+  a.oct_start = w.oct_start;
+  a.kalloc = w.counts + w.seg_cap;
+  a.k_host = 0;
+  a.ctrl = c->ctrl;
+  a.codes = w.keys_b;
+  a.prefix_n = w.prefix_n;
+  a.has_leaf_left = w.hl;
+  a.has_leaf_right = w.hr;
+  a.left_child = w.lc;
+  a.parent = w.par;
+  a.edge = w.edge;
+  a.offsets = w.offs;
+  a.n_unique = w.counts;
+  a.n_brt_host = 0;
+  return a;
+}
+
+void enqueue_octree(ocp_ctx c, const Plan& p) {
+  Workspace& w = c->ws;
+  hipStream_t s = c->stream;
+  int32_t* kalloc = w.counts + w.seg_cap;
+  launch_oct_offsets(kalloc, p.n_seg, w.oct_start, c->ctrl, w.k_cap, s);
+  launch_zero_nodes(w.nodes, c->ctrl, w.k_cap, s);
+  const OctArgs a = pipeline_oct_args(c);
+  launch_oct_scatter(a, p.sv, p.ts_node, p.tiles_node, 1 | 2, s);
+  launch_oct_finalize(a, p.sv, p.ts_node, p.tiles_node, 1 | 2 | 4 | 8, s);
+}
+
+void mark(ocp_ctx c, int k, bool capturing) {
+  if (c->stage_timing && !capturing) (void)hipEventRecord(c->ev[k], c->stream);
+}
+
+// The whole chain, enqueued on the context stream.
+void enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
+                      bool capturing) {
+  Workspace& w = c->ws;
+  hipStream_t s = c->stream;
+  const TuneTable& tt = c->tune;
+  int32_t* n_unique = w.counts;
+  int32_t* kalloc = w.counts + w.seg_cap;
+  mark(c, 0, capturing);
+  launch_begin(c->ctrl, w.hist, int64_t(p.n_seg) * 1024, s);
+  launch_morton(pts, f3, w.keys_a, p.sv, p.ts_morton, p.tiles_morton, w.hist, tt, s);
+  mark(c, 1, capturing);
+  uint32_t* src = w.keys_a;
+  uint32_t* dst = w.keys_b;
+  for (int pass = 0; pass < 4; ++pass) {
+    launch_onesweep_pass(src, dst, p.sv, p.ts_sort, p.tiles_sort, w.hist, w.flags,
+                         c->ctrl, pass, tt, s);
+    std::swap(src, dst);
+  }
+  mark(c, 2, capturing);
+  launch_fill_u32(reinterpret_cast<uint32_t*>(w.counts), 0u, int64_t(w.seg_cap) * 2, s);
+  launch_unique(w.keys_a, w.keys_b, p.sv, p.ts_scan, p.tiles_scan, w.flags, c->ctrl,
+                n_unique, kalloc, tt, s);
+  mark(c, 3, capturing);
+  launch_radix_tree(w.keys_b, w.prefix_n, w.hl, w.hr, w.lc, w.par, p.sv, p.ts_node,
+                    p.tiles_node, n_unique, 0, s);
+  mark(c, 4, capturing);
+  launch_edge_scan(w.prefix_n, w.par, w.edge, w.offs, p.sv, p.ts_scan, p.tiles_scan,
+                   n_unique, 0, w.flags, c->ctrl, kalloc, 0, tt, s);
+  mark(c, 5, capturing);
+  enqueue_octree(c, p);
+  mark(c, 6, capturing);
+}
+
+std::string graph_key(const char* kind, const float* pts, int f3, int64_t n,
+                      float mn, float rg, const std::vector<int32_t>* offs,
+                      const TuneTable& tt) {
+  char buf[256];
+  std::snprintf(buf, sizeof(buf), "%s|%p|%d|%lld|%a|%a|%d", kind,
+                static_cast<const void*>(pts), f3, static_cast<long long>(n), mn, rg,
+                tune_key(tt));
+  std::string k(buf);
+  if (offs) {
+    uint64_t h = 1469598103934665603ull;
+    for (int32_t v : *offs) h = (h ^ static_cast<uint32_t>(v)) * 1099511628211ull;
+    k += "|" + std::to_string(offs->size()) + "|" + std::to_string(h);
+  }
+  return k;
+}
+
+int run_plan(ocp_ctx c, const float* pts, int f3, const Plan& p,
+             const std::string& key) {
+  if (c->graph_mode) {
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+      hipGraph_t g = nullptr;
+      OCP_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_pipeline(c, pts, f3 != 0, p, true);
+      hipError_t e = hipGetLastError();
+      hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+      if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(c, OCP_ERR_HIP, std::string("graph capture: ") +
+                                        hipGetErrorString(e != hipSuccess ? e : e2));
+      }
+      hipGraphExec_t ex = nullptr;
+      e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return fail(c, OCP_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+      it = c->graphs.emplace(key, ex).first;
+    }
+    OCP_HIP(c, hipGraphLaunch(it->second, c->stream));
+    c->ev_recorded = false;
+  } else {
+    enqueue_pipeline(c, pts, f3 != 0, p, false);
+    OCP_CHECK_LAUNCH(c);
+    c->ev_recorded = c->stage_timing;
+  }
+  c->last_plan = p;
+  c->have_result = true;
+  return OCP_OK;
+}
+
+// After a build: if the octree did not fit, grow the node buffer and redo
+// stage 7 (its inputs are intact). Synchronises.
+int settle(ocp_ctx c) {
+  if (!c->have_result) return fail(c, OCP_ERR_STATE, "no build has run");
+  uint32_t hc[kCtrlWords];
+  OCP_HIP(c, hipMemcpyAsync(hc, c->ctrl, sizeof(hc), hipMemcpyDeviceToHost, c->stream));
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  if (hc[kCtrlOctOk] == 0) {
+    const int64_t need = hc[kCtrlOctTotal];
+    clear_graphs(c);
+    if (int r = alloc_nodes(c, need + need / 4 + 1024)) return r;
+    enqueue_octree(c, c->last_plan);
+    OCP_CHECK_LAUNCH(c);
+    OCP_HIP(c, hipMemcpyAsync(hc, c->ctrl, sizeof(hc), hipMemcpyDeviceToHost, c->stream));
+    OCP_HIP(c, hipStreamSynchronize(c->stream));
+    if (hc[kCtrlOctOk] == 0) return fail(c, OCP_ERR_CAPACITY, "octree node buffer");
+  }
+  return OCP_OK;
+}
+
+int check_ctx(ocp_ctx c) { return c ? OCP_OK : OCP_ERR_INVALID; }
+
+}  // namespace
+
+// ======================================================================
+extern "C" {
+
+int ocp_create(ocp_ctx* out, int device, void* stream) {
+  if (!out) return OCP_ERR_INVALID;
+  *out = nullptr;
+  auto* c = new ocp_context();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete c;
+    return OCP_ERR_HIP;
+  }
+  if (stream) {
+    c->stream = static_cast<hipStream_t>(stream);
+  } else {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      return OCP_ERR_HIP;
+    }
+    c->own_stream = true;
+  }
+  if (hipMalloc(reinterpret_cast<void**>(&c->ctrl), sizeof(uint32_t) * kCtrlWords) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&c->s_hist), sizeof(uint32_t) * 1024) != hipSuccess ||
+      hipMemsetAsync(c->ctrl, 0, sizeof(uint32_t) * kCtrlWords, c->stream) != hipSuccess) {
+    ocp_destroy(c);
+    return OCP_ERR_HIP;
+  }
+  for (auto& e : c->ev) (void)hipEventCreate(&e);
+  *out = c;
+  return OCP_OK;
+}
+
+int ocp_destroy(ocp_ctx c) {
+  if (!c) return OCP_ERR_INVALID;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  clear_graphs(c);
+  free_workspace(c->ws);
+  dev_free(c->ctrl);
+  dev_free(c->s_flags);
+  dev_free(c->s_hist);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return OCP_OK;
+}
+
+const char* ocp_last_error(ocp_ctx c) { return c ? c->err.c_str() : "null context"; }
+
+int ocp_synchronize(ocp_ctx c) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  return OCP_OK;
+}
+
+int ocp_malloc(ocp_ctx c, void** ptr, size_t bytes) {
+  if (check_ctx(c) || !ptr) return OCP_ERR_INVALID;
+  OCP_HIP(c, hipMalloc(ptr, bytes ? bytes : 1));
+  return OCP_OK;
+}
+int ocp_free(ocp_ctx c, void* ptr) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  if (ptr) OCP_HIP(c, hipFree(ptr));
+  return OCP_OK;
+}
+int ocp_memcpy_h2d(ocp_ctx c, void* dst, const void* src, size_t bytes) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (!bytes) return OCP_OK;
+  OCP_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  return OCP_OK;
+}
+int ocp_memcpy_d2h(ocp_ctx c, void* dst, const void* src, size_t bytes) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (!bytes) return OCP_OK;
+  OCP_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  return OCP_OK;
+}
+int ocp_memset(ocp_ctx c, void* dst, int value, size_t bytes) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (!bytes) return OCP_OK;
+  OCP_HIP(c, hipMemsetAsync(dst, value, bytes, c->stream));
+  return OCP_OK;
+}
+
+// ---------------------------------------------------------------- stages
+static int morton_common(ocp_ctx c, const float* data, uint32_t* keys, size_t n,
+                         float mn, float rg, bool f3) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (n == 0) return OCP_OK;
+  if (!data || !keys) return fail(c, OCP_ERR_INVALID, "null pointer");
+  if (static_cast<int64_t>(n) > kMaxPoints) return fail(c, OCP_ERR_CAPACITY, "n > 2^30-1");
+  if (!f3 && (reinterpret_cast<uintptr_t>(data) & 15u))
+    return fail(c, OCP_ERR_INVALID, "float4 input must be 16-byte aligned");
+  const SegView sv = single_view(static_cast<int64_t>(n), mn, rg);
+  launch_morton(data, f3, keys, sv, nullptr,
+                static_cast<int32_t>(cdiv(static_cast<int64_t>(n), morton_tile(c->tune))),
+                nullptr, c->tune, c->stream);
+  OCP_CHECK_LAUNCH(c);
+  return OCP_OK;
+}
+
+int ocp_k_ComputeMortonCode(ocp_ctx c, const float* data, uint32_t* keys, size_t n,
+                            float mn, float rg) {
+  return morton_common(c, data, keys, n, mn, rg, false);
+}
+int ocp_k_ComputeMortonCode3(ocp_ctx c, const float* data, uint32_t* keys, size_t n,
+                             float mn, float rg) {
+  return morton_common(c, data, keys, n, mn, rg, true);
+}
+
+int ocp_k_GlobalHistogram(ocp_ctx c, const uint32_t* sort, uint32_t* hist, uint32_t size) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (!hist || (size && !sort)) return fail(c, OCP_ERR_INVALID, "null pointer");
+  if (size > kMaxPoints) return fail(c, OCP_ERR_CAPACITY, "size > 2^30-1");
+  launch_fill_u32(hist, 0u, 1024, c->stream);
+  const SegView sv = single_view(size, 0.0f, 1.0f);
+  launch_histogram(sort, sv, nullptr, static_cast<int32_t>(cdiv(size, morton_tile(c->tune))),
+                   hist, c->tune, c->stream);
+  OCP_CHECK_LAUNCH(c);
+  return OCP_OK;
+}
+
+int ocp_k_DigitBinning(ocp_ctx c, const uint32_t* hist, const uint32_t* sort, uint32_t* alt,
+                       uint32_t size, uint32_t radix_shift) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (size == 0) return OCP_OK;
+  if (!hist || !sort || !alt) return fail(c, OCP_ERR_INVALID, "null pointer");
+  if (radix_shift > 24 || (radix_shift & 7u)) return fail(c, OCP_ERR_INVALID, "radix_shift must be 0, 8, 16 or 24");
+  if (size > kMaxPoints) return fail(c, OCP_ERR_CAPACITY, "size > 2^30-1");
+  const int32_t tiles = static_cast<int32_t>(cdiv(size, sort_tile(c->tune)));
+  if (int r = ensure_scratch(c, int64_t(tiles) * 256)) return r;
+  launch_begin(c->ctrl, nullptr, 0, c->stream);
+  launch_onesweep_pass(sort, alt, single_view(size, 0.0f, 1.0f), nullptr, tiles, hist,
+                       c->s_flags, c->ctrl, static_cast<int>(radix_shift >> 3), c->tune,
+                       c->stream);
+  OCP_CHECK_LAUNCH(c);
+  return OCP_OK;
+}
+
+int ocp_k_SortKeys(ocp_ctx c, uint32_t* keys, uint32_t* alt, uint32_t n) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (n <= 1) return OCP_OK;
+  if (!keys || !alt) return fail(c, OCP_ERR_INVALID, "null pointer");
+  if (n > kMaxPoints) return fail(c, OCP_ERR_CAPACITY, "n > 2^30-1");
+  const int32_t tiles = static_cast<int32_t>(cdiv(n, sort_tile(c->tune)));
+  if (int r = ensure_scratch(c, int64_t(tiles) * 256)) return r;
+  const SegView sv = single_view(n, 0.0f, 1.0f);
+  launch_begin(c->ctrl, c->s_hist, 1024, c->stream);
+  launch_histogram(keys, sv, nullptr, static_cast<int32_t>(cdiv(n, morton_tile(c->tune))),
+                   c->s_hist, c->tune, c->stream);
+  uint32_t* src = keys;
+  uint32_t* dst = alt;
+  for (int pass = 0; pass < 4; ++pass) {
+    launch_onesweep_pass(src, dst, sv, nullptr, tiles, c->s_hist, c->s_flags, c->ctrl, pass,
+                         c->tune, c->stream);
+    std::swap(src, dst);
+  }
+  OCP_CHECK_LAUNCH(c);
+  return OCP_OK;
+}
+
+int ocp_k_CountUnique(ocp_ctx c, uint32_t* keys, int32_t* num_unique_out, int32_t n) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (!num_unique_out || (n > 0 && !keys)) return fail(c, OCP_ERR_INVALID, "null pointer");
+  if (n < 0) return fail(c, OCP_ERR_INVALID, "n < 0");
+  if (n > kMaxPoints) return fail(c, OCP_ERR_CAPACITY, "n > 2^30-1");
+  launch_fill_u32(reinterpret_cast<uint32_t*>(num_unique_out), 0u, 1, c->stream);
+  if (n == 0) {
+    OCP_CHECK_LAUNCH(c);
+    return OCP_OK;
+  }
+  const int32_t tiles = static_cast<int32_t>(cdiv(n, scan_tile(c->tune)));
+  if (int r = ensure_scratch(c, tiles)) return r;
+  launch_begin(c->ctrl, nullptr, 0, c->stream);
+  launch_unique(keys, keys, single_view(n, 0.0f, 1.0f), nullptr, tiles, c->s_flags, c->ctrl,
+                num_unique_out, nullptr, c->tune, c->stream);
+  OCP_CHECK_LAUNCH(c);
+  return OCP_OK;
+}
+
+int ocp_k_BuildRadixTree(ocp_ctx c, int32_t n, const uint32_t* codes, uint8_t* prefix_n,
+                         uint8_t* hl, uint8_t* hr, int32_t* lc, int32_t* parent) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (n <= 0) return OCP_OK;
+  if (!codes || !prefix_n || !hl || !hr || !lc || !parent)
+    return fail(c, OCP_ERR_INVALID, "null pointer");
+  if (n > kMaxPoints) return fail(c, OCP_ERR_CAPACITY, "n > 2^30-1");
+  launch_radix_tree(codes, prefix_n, hl, hr, lc, parent, single_view(n + 1, 0.0f, 1.0f),
+                    nullptr, static_cast<int32_t>(cdiv(n, kNodeTile)), nullptr, n, c->stream);
+  OCP_CHECK_LAUNCH(c);
+  return OCP_OK;
+}
+
+int ocp_k_EdgeCount(ocp_ctx c, const uint8_t* prefix_n, const int32_t* parents,
+                    int32_t* edge_count, int32_t n) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (n <= 0) return OCP_OK;
+  if (!prefix_n || !parents || !edge_count) return fail(c, OCP_ERR_INVALID, "null pointer");
+  launch_edge_scan(prefix_n, parents, edge_count, nullptr, single_view(n, 0.0f, 1.0f),
+                   nullptr, static_cast<int32_t>(cdiv(n, scan_tile(c->tune))), nullptr, n,
+                   nullptr, c->ctrl, nullptr, 1, c->tune, c->stream);
+  OCP_CHECK_LAUNCH(c);
+  return OCP_OK;
+}
+
+int ocp_k_PartialSum(ocp_ctx c, const int32_t* data, int64_t first, int64_t last,
+                     int32_t* d_first) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  const int64_t n = last - first;
+  if (n <= 0) return OCP_OK;
+  if (!data || !d_first || first < 0) return fail(c, OCP_ERR_INVALID, "bad arguments");
+  if (n > kMaxPoints) return fail(c, OCP_ERR_CAPACITY, "n > 2^30-1");
+  const int32_t tiles = static_cast<int32_t>(cdiv(n, scan_tile(c->tune)));
+  if (int r = ensure_scratch(c, tiles)) return r;
+  launch_begin(c->ctrl, nullptr, 0, c->stream);
+  launch_edge_scan(nullptr, nullptr, const_cast<int32_t*>(data + first), d_first,
+                   single_view(n, 0.0f, 1.0f), nullptr, tiles, nullptr,
+                   static_cast<int32_t>(n), c->s_flags, c->ctrl, nullptr, 2, c->tune,
+                   c->stream);
+  OCP_CHECK_LAUNCH(c);
+  return OCP_OK;
+}
+
+int ocp_k_InitOctNodes(ocp_ctx c, ocp_oct_node* nodes, int32_t n_alloc, const uint32_t* codes,
+                       const uint8_t* prefix_n, int32_t n_brt, float mn, float rg) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (n_alloc <= 0) return OCP_OK;
+  if (!nodes || !codes || (n_brt > 0 && !prefix_n)) return fail(c, OCP_ERR_INVALID, "null pointer");
+  launch_fill_u32(reinterpret_cast<uint32_t*>(nodes), 0u,
+                  int64_t(n_alloc) * (sizeof(OctNode) / 4), c->stream);
+  launch_oct_root(reinterpret_cast<OctNode*>(nodes), codes, prefix_n, n_brt, mn, rg, c->stream);
+  OCP_CHECK_LAUNCH(c);
+  return OCP_OK;
+}
+
+static OctArgs stage_oct_args(ocp_oct_node* nodes, const int32_t* offs, const int32_t* counts,
+                              const uint32_t* codes, const uint8_t* pn, const uint8_t* hl,
+                              const uint8_t* hr, const int32_t* par, const int32_t* lc,
+                              int32_t n, int32_t n_alloc) {
+  OctArgs a{};
+  a.nodes = reinterpret_cast<OctNode*>(nodes);
+  a.k_host = n_alloc;
+  a.codes = codes;
+  a.prefix_n = pn;
+  a.has_leaf_left = hl;
+  a.has_leaf_right = hr;
+  a.left_child = lc;
+  a.parent = par;
+  a.edge = counts;
+  a.offsets = offs;
+  a.n_brt_host = n;
+  return a;
+}
+
+int ocp_k_MakeOctNodes(ocp_ctx c, ocp_oct_node* nodes, const int32_t* offs,
+                       const int32_t* counts, const uint32_t* codes, const uint8_t* pn,
+                       const int32_t* par, float mn, float rg, int32_t n, int32_t n_alloc) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (n <= 0 || n_alloc <= 0) return OCP_OK;
+  if (!nodes || !offs || !counts || !codes || !pn || !par)
+    return fail(c, OCP_ERR_INVALID, "null pointer");
+  if (n >= kMaxSegPoints) return fail(c, OCP_ERR_CAPACITY, "N >= 2^27");
+  const OctArgs a = stage_oct_args(nodes, offs, counts, codes, pn, nullptr, nullptr, par,
+                                   nullptr, n, n_alloc);
+  const SegView sv = single_view(n + 1, mn, rg);
+  const int32_t tiles = static_cast<int32_t>(cdiv(n, kNodeTile));
+  launch_oct_scatter(a, sv, nullptr, tiles, 1, c->stream);
+  launch_oct_finalize(a, sv, nullptr, tiles, 1 | 4, c->stream);
+  OCP_CHECK_LAUNCH(c);
+  return OCP_OK;
+}
+
+int ocp_k_LinkLeafNodes(ocp_ctx c, ocp_oct_node* nodes, const int32_t* offs,
+                        const int32_t* counts, const uint32_t* codes, const uint8_t* hl,
+                        const uint8_t* hr, const uint8_t* pn, const int32_t* par,
+                        const int32_t* lc, int32_t n, int32_t n_alloc) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (n <= 0 || n_alloc <= 0) return OCP_OK;
+  if (!nodes || !offs || !counts || !codes || !hl || !hr || !pn || !par || !lc)
+    return fail(c, OCP_ERR_INVALID, "null pointer");
+  if (n >= kMaxSegPoints) return fail(c, OCP_ERR_CAPACITY, "N >= 2^27");
+  const OctArgs a = stage_oct_args(nodes, offs, counts, codes, pn, hl, hr, par, lc, n, n_alloc);
+  const SegView sv = single_view(n + 1, 0.0f, 1.0f);
+  const int32_t tiles = static_cast<int32_t>(cdiv(n, kNodeTile));
+  launch_oct_scatter(a, sv, nullptr, tiles, 2 | 4, c->stream);
+  launch_oct_finalize(a, sv, nullptr, tiles, 2, c->stream);
+  OCP_CHECK_LAUNCH(c);
+  return OCP_OK;
+}
+
+// ---------------------------------------------------------------- pipeline
+int ocp_build(ocp_ctx c, const float* points, int float3, int64_t n, float mn, float rg) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (n < 0) return fail(c, OCP_ERR_INVALID, "n < 0");
+  if (n > 0 && !points) return fail(c, OCP_ERR_INVALID, "null points");
+  if (n > kMaxSegPoints) return fail(c, OCP_ERR_CAPACITY, "a single cloud holds at most 2^27 points");
+  if (!(rg > 0.0f)) return fail(c, OCP_ERR_INVALID, "range must be > 0");
+  if (!float3 && (reinterpret_cast<uintptr_t>(points) & 15u))
+    return fail(c, OCP_ERR_INVALID, "float4 input must be 16-byte aligned");
+  const TuneTable& tt = c->tune;
+  if (int r = ensure_workspace(c, n, 1, flag_words_for(tt, n, 1))) return r;
+  const Plan p = plain_plan(tt, n, mn, rg);
+  c->last.valid = true;
+  c->last.batched = false;
+  c->last.pts = points;
+  c->last.float3 = float3;
+  c->last.n = n;
+  c->last.mn = mn;
+  c->last.rg = rg;
+  return run_plan(c, points, float3, p, graph_key("plain", points, float3, n, mn, rg, nullptr, tt));
+}
+
+int ocp_build_batched(ocp_ctx c, const float* points, int float3, const int32_t* offsets,
+                      const float* smin, const float* srg, int32_t n_seg) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  Plan p;
+  if (int r = batched_plan(c, offsets, smin, srg, n_seg, &p)) return r;
+  if (p.n_total > 0 && !points) return fail(c, OCP_ERR_INVALID, "null points");
+  if (!float3 && (reinterpret_cast<uintptr_t>(points) & 15u))
+    return fail(c, OCP_ERR_INVALID, "float4 input must be 16-byte aligned");
+  c->last.valid = true;
+  c->last.batched = true;
+  c->last.pts = points;
+  c->last.float3 = float3;
+  c->last.offsets.assign(offsets, offsets + n_seg + 1);
+  c->last.seg_min.assign(smin, smin + n_seg);
+  c->last.seg_range.assign(srg, srg + n_seg);
+  return run_plan(c, points, float3, p,
+                  graph_key("batch", points, float3, p.n_total, 0, 0, &c->last.offsets, c->tune));
+}
+
+int ocp_set_graph_mode(ocp_ctx c, int enable) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  c->graph_mode = enable != 0;
+  return OCP_OK;
+}
+
+int ocp_get_results(ocp_ctx c, ocp_results* out) {
+  if (check_ctx(c) || !out) return OCP_ERR_INVALID;
+  if (int r = settle(c)) return r;
+  const Workspace& w = c->ws;
+  out->sorted_keys = w.keys_a;
+  out->unique_keys = w.keys_b;
+  out->prefix_n = w.prefix_n;
+  out->has_leaf_left = w.hl;
+  out->has_leaf_right = w.hr;
+  out->left_child = w.lc;
+  out->parent = w.par;
+  out->edge_count = w.edge;
+  out->offsets = w.offs;
+  out->nodes = reinterpret_cast<const ocp_oct_node*>(w.nodes);
+  out->n_unique = w.counts;
+  out->n_oct = w.counts + w.seg_cap;
+  out->oct_start = w.oct_start;
+  out->n_seg = c->last_plan.n_seg;
+  out->n_points = c->last_plan.n_total;
+  return OCP_OK;
+}
+
+int ocp_get_counts(ocp_ctx c, int32_t* n_unique, int32_t* n_oct) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (int r = settle(c)) return r;
+  const Workspace& w = c->ws;
+  const int32_t ns = c->last_plan.n_seg;
+  if (n_unique)
+    OCP_HIP(c, hipMemcpy(n_unique, w.counts, sizeof(int32_t) * ns, hipMemcpyDeviceToHost));
+  if (n_oct)
+    OCP_HIP(c, hipMemcpy(n_oct, w.counts + w.seg_cap, sizeof(int32_t) * ns, hipMemcpyDeviceToHost));
+  return OCP_OK;
+}
+
+int ocp_set_stage_timing(ocp_ctx c, int enable) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  c->stage_timing = enable != 0;
+  return OCP_OK;
+}
+
+int ocp_get_stage_ms(ocp_ctx c, float* ms7) {
+  if (check_ctx(c) || !ms7) return OCP_ERR_INVALID;
+  if (!c->ev_recorded) return fail(c, OCP_ERR_STATE, "no eager build with stage timing on");
+  OCP_HIP(c, hipEventSynchronize(c->ev[6]));
+  for (int k = 0; k < 6; ++k) OCP_HIP(c, hipEventElapsedTime(&ms7[k], c->ev[k], c->ev[k + 1]));
+  OCP_HIP(c, hipEventElapsedTime(&ms7[6], c->ev[0], c->ev[6]));
+  return OCP_OK;
+}
+
+int ocp_time_builds(ocp_ctx c, const float* points, int float3, const int32_t* offsets,
+                    const float* smin, const float* srg, int32_t n_seg, int warmup, int steps,
+                    float* ms_per_build) {
+  if (check_ctx(c) || !ms_per_build || steps < 1) return OCP_ERR_INVALID;
+  // n_seg == 0: plain build of offsets[1] points with box (smin[0], srg[0]).
+  auto once = [&]() -> int {
+    if (n_seg == 0) return ocp_build(c, points, float3, offsets[1], smin[0], srg[0]);
+    return ocp_build_batched(c, points, float3, offsets, smin, srg, n_seg);
+  };
+  if (int r = once()) return r;
+  if (int r = settle(c)) return r;  // node capacity fixed before timing
+  for (int k = 0; k < warmup; ++k)
+    if (int r = once()) return r;
+  hipEvent_t a, b;
+  OCP_HIP(c, hipEventCreate(&a));
+  OCP_HIP(c, hipEventCreate(&b));
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  OCP_HIP(c, hipEventRecord(a, c->stream));
+  for (int k = 0; k < steps; ++k)
+    if (int r = once()) return r;
+  OCP_HIP(c, hipEventRecord(b, c->stream));
+  OCP_HIP(c, hipEventSynchronize(b));
+  float ms = 0;
+  OCP_HIP(c, hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms_per_build = ms / steps;
+  return settle(c);
+}
+
+// ---------------------------------------------------------------- tuning
+int ocp_get_tune(ocp_ctx c, ocp_tune* out) {
+  if (check_ctx(c) || !out) return OCP_ERR_INVALID;
+  out->sort_waves = c->tune.sort_waves;
+  out->sort_kpt = c->tune.sort_kpt;
+  out->morton_ppt = c->tune.morton_ppt;
+  out->morton_grid = c->tune.morton_grid;
+  out->scan_ipt = c->tune.scan_ipt;
+  return OCP_OK;
+}
+
+int ocp_set_tune(ocp_ctx c, const ocp_tune* in) {
+  if (check_ctx(c) || !in) return OCP_ERR_INVALID;
+  const bool sort_ok = sort_config_supported(in->sort_waves, in->sort_kpt) ||
+                       (in->sort_waves == 8 && in->sort_kpt == 15);
+  if (!sort_ok || !morton_config_supported(in->morton_ppt) ||
+      !scan_config_supported(in->scan_ipt) || in->morton_grid < 1)
+    return fail(c, OCP_ERR_INVALID, "unsupported tile configuration");
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  c->tune.sort_waves = in->sort_waves;
+  c->tune.sort_kpt = in->sort_kpt;
+  c->tune.morton_ppt = in->morton_ppt;
+  c->tune.morton_grid = in->morton_grid;
+  c->tune.scan_ipt = in->scan_ipt;
+  clear_graphs(c);
+  c->tab_offsets.clear();
+  return OCP_OK;
+}
+
+int ocp_autotune(ocp_ctx c, const float* points, int float3, int64_t n,
+                 const int32_t* offsets, const float* smin, const float* srg, int32_t n_seg,
+                 int reps, ocp_tune* chosen) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (reps < 1) reps = 3;
+  std::vector<int32_t> plain_off = {0, static_cast<int32_t>(n)};
+  const int32_t* offs = n_seg == 0 ? plain_off.data() : offsets;
+  auto time_cfg = [&](const ocp_tune& t, float* ms) -> int {
+    if (int r = ocp_set_tune(c, &t)) return r;
+    return ocp_time_builds(c, points, float3, offs, smin, srg, n_seg, 1, reps, ms);
+  };
+  ocp_tune best;
+  ocp_get_tune(c, &best);
+  float best_ms = 0;
+  if (int r = time_cfg(best, &best_ms)) return r;
+  // Coordinate descent: sort tile, then scan tile, then Morton tile/grid.
+  const int sort_cfgs[][2] = {{8, 15}, {4, 16}, {4, 24}, {8, 8}, {8, 12}, {8, 16}, {16, 8}};
+  for (const auto& sc : sort_cfgs) {
+    ocp_tune t = best;
+    t.sort_waves = sc[0];
+    t.sort_kpt = sc[1];
+    float ms;
+    if (int r = time_cfg(t, &ms)) return r;
+    if (ms < best_ms) { best_ms = ms; best = t; }
+  }
+  for (int ipt : {4, 8, 16}) {
+    ocp_tune t = best;
+    t.scan_ipt = ipt;
+    float ms;
+    if (int r = time_cfg(t, &ms)) return r;
+    if (ms < best_ms) { best_ms = ms; best = t; }
+  }
+  for (int ppt : {4, 8, 16, 32}) {
+    for (int grid : {512, 1024, 2048}) {
+      ocp_tune t = best;
+      t.morton_ppt = ppt;
+      t.morton_grid = grid;
+      float ms;
+      if (int r = time_cfg(t, &ms)) return r;
+      if (ms < best_ms) { best_ms = ms; best = t; }
+    }
+  }
+  if (int r = ocp_set_tune(c, &best)) return r;
+  if (chosen) *chosen = best;
+  return OCP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,150 @@
+// Stages 5 + 6: edge count (kernels/05_edge_count.cpp:3-16:
+// e[0] = 0, e[i] = prefix_n[i]/3 - prefix_n[parent[i]]/3) fused with the
+// inclusive scan of kernels/06_partial_sum.cpp:3-15 (the reference GPU path
+// has no scan and round-trips to the host, apps/gpu_demo/main.cu:308-310).
+//
+// Tiles of 256*IPT nodes, ticket-ordered, one decoupled-lookback chain per
+// segment. Each thread handles one node per row of 256 (coalesced loads of
+// prefix_n and parent, one gathered byte prefix_n[parent]); rows are scanned
+// with wave64 shuffles, one wave scans the 4*IPT row totals, one thread
+// looks back. Each tile also reduces max(P[i] + e[i]) into the segment's
+// octree slot count with one atomicMax.
+//
+// Modes: 0 = fused, 1 = edge count only, 2 = scan of `edge` only.
+#include "ocp_internal.hpp"
+
+namespace ocp {
+namespace {
+
+template <int IPT>
+__global__ __launch_bounds__(256) void edge_scan_kernel(
+    const uint8_t* __restrict__ prefix_all, const int32_t* __restrict__ par_all,
+    int32_t* __restrict__ edge_all, int32_t* __restrict__ off_all, SegView sv,
+    const int32_t* __restrict__ tile_start, const int32_t* __restrict__ n_unique,
+    int32_t n_brt_host, uint64_t* __restrict__ flags,
+    uint32_t* __restrict__ ctrl, int32_t* __restrict__ kalloc, int mode) {
+  constexpr int THREADS = 256;
+  constexpr int NW = THREADS / 64;
+  constexpr int T = THREADS * IPT;
+  static_assert(IPT * NW <= 64, "row totals are scanned by one wave");
+  __shared__ uint32_t s_rows[IPT * NW];
+  __shared__ int32_t s_max[NW];
+  __shared__ int32_t s_ticket;
+  __shared__ uint32_t s_excl;
+
+  const int tid = threadIdx.x;
+  const uint32_t w = wave_id();
+  const uint32_t lane = lane_id();
+  const bool scan = mode != 1;
+  int32_t t = blockIdx.x;
+  if (scan) {
+    if (tid == 0) s_ticket = static_cast<int32_t>(atomicAdd(&ctrl[kCtrlTicket0 + kStageScan], 1u));
+    __syncthreads();
+    t = s_ticket;
+  }
+  const TileRef r = find_tile(tile_start, sv.n_seg, t);
+  const int32_t n = n_unique ? n_unique[r.seg] - 1 : n_brt_host;
+  const int32_t base = r.local * T;
+  if (base >= n) return;  // no later tile of this chain waits on it
+  const int32_t cnt = min(T, n - base);
+  const int32_t sb = seg_begin(sv, r.seg);
+  const uint8_t* pn = prefix_all + sb;
+  int32_t* edge = edge_all + sb;
+
+  int32_t e[IPT];
+#pragma unroll
+  for (int k = 0; k < IPT; ++k) {
+    const int32_t i = base + k * THREADS + tid;
+    int32_t v = 0;
+    if (k * THREADS + tid < cnt) {
+      if (mode == 2) {
+        v = edge[i];
+      } else if (i > 0) {
+        const int32_t p = par_all[sb + i];
+        v = static_cast<int32_t>(pn[i] / 3) - static_cast<int32_t>(pn[p] / 3);
+      }
+      if (mode != 2) edge[i] = v;
+    }
+    e[k] = v;
+  }
+  if (!scan) return;
+
+  uint32_t incl[IPT];
+#pragma unroll
+  for (int k = 0; k < IPT; ++k) {
+    incl[k] = wave_incl_scan(static_cast<uint32_t>(e[k]));
+    if (lane == 63) s_rows[k * NW + w] = incl[k];
+  }
+  __syncthreads();
+  if (w == 0) {
+    const uint32_t v = lane < IPT * NW ? s_rows[lane] : 0u;
+    const uint32_t in = wave_incl_scan(v);
+    if (lane < IPT * NW) s_rows[lane] = in - v;
+    const uint32_t total = __shfl(in, IPT * NW - 1, 64);
+    if (lane == 0) {
+      const uint32_t tag = ctrl[kCtrlEpoch] * 8u + kStageScan;
+      uint32_t prev = 0;
+      if (r.local == 0) {
+        flag_store(&flags[t], flag_pack(tag, kFlagInc, total));
+      } else {
+        flag_store(&flags[t], flag_pack(tag, kFlagAgg, total));
+        prev = lookback(flags, 1, t, tag);
+        flag_store(&flags[t], flag_pack(tag, kFlagInc, prev + total));
+      }
+      s_excl = prev;
+    }
+  }
+  __syncthreads();
+  const uint32_t excl = s_excl;
+  int32_t* off = off_all + sb;
+  int32_t kmax = 1;
+#pragma unroll
+  for (int k = 0; k < IPT; ++k) {
+    const int32_t idx = k * THREADS + tid;
+    if (idx < cnt) {
+      const int32_t p = static_cast<int32_t>(excl + s_rows[k * NW + w] + incl[k]);
+      off[base + idx] = p;
+      kmax = max(kmax, p + e[k]);
+    }
+  }
+  if (kalloc) {
+    kmax = wave_incl_max(kmax);
+    if (lane == 0) s_max[w] = kmax;
+    __syncthreads();
+    if (tid == 0) {
+      int32_t m = s_max[0];
+#pragma unroll
+      for (int q = 1; q < NW; ++q) m = max(m, s_max[q]);
+      atomicMax(&kalloc[r.seg], m);
+    }
+  }
+}
+
+template <int IPT>
+void launch_ipt(const uint8_t* prefix_n, const int32_t* parent, int32_t* edge,
+                int32_t* offsets, const SegView& sv, const int32_t* tile_start,
+                int32_t total_tiles, const int32_t* n_unique,
+                int32_t n_brt_host, uint64_t* flags, uint32_t* ctrl,
+                int32_t* kalloc, int mode, hipStream_t s) {
+  edge_scan_kernel<IPT><<<total_tiles, 256, 0, s>>>(
+      prefix_n, parent, edge, offsets, sv, tile_start, n_unique, n_brt_host,
+      flags, ctrl, kalloc, mode);
+}
+
+}  // namespace
+
+void launch_edge_scan(const uint8_t* prefix_n, const int32_t* parent,
+                      int32_t* edge, int32_t* offsets, const SegView& sv,
+                      const int32_t* tile_start, int32_t total_tiles,
+                      const int32_t* n_unique, int32_t n_brt_host,
+                      uint64_t* flags, uint32_t* ctrl, int32_t* kalloc,
+                      int mode, const TuneTable& tt, hipStream_t s) {
+  if (total_tiles <= 0) return;
+  switch (tt.scan_ipt) {
+    case 4: launch_ipt<4>(prefix_n, parent, edge, offsets, sv, tile_start, total_tiles, n_unique, n_brt_host, flags, ctrl, kalloc, mode, s); break;
+    case 8: launch_ipt<8>(prefix_n, parent, edge, offsets, sv, tile_start, total_tiles, n_unique, n_brt_host, flags, ctrl, kalloc, mode, s); break;
+    default: launch_ipt<16>(prefix_n, parent, edge, offsets, sv, tile_start, total_tiles, n_unique, n_brt_host, flags, ctrl, kalloc, mode, s); break;
+  }
+}
+
+}  // namespace ocp

@@ -1,0 +1,120 @@
+// Stage 1: Morton encode (kernels/01_morton.cpp:5-14, include/shared/
+// morton.h:38-46), fused with the four 8-bit digit histograms that the
+// OneSweep passes need (the role of k_GlobalHistogram, kernels/cuda/
+// 02_sort.cu:208-338), so the keys are read from HBM once less.
+//
+// Persistent workgroups of 256 threads walk a contiguous run of tiles
+// (256 * PPT points each, never straddling a segment). Points arrive as
+// 16-byte float4 loads (or three 4-byte loads for the float3 variant), keys
+// leave as coalesced 4-byte stores. Digit counts accumulate in a 4 x 256 LDS
+// histogram that is flushed with global atomics (zero bins skipped) when the
+// run changes segment and at its end.
+#include "ocp_internal.hpp"
+
+namespace ocp {
+namespace {
+
+template <int PPT, bool F3, bool HIST>
+__global__ __launch_bounds__(256) void morton_kernel(
+    const float* __restrict__ pts, uint32_t* __restrict__ keys, SegView sv,
+    const int32_t* __restrict__ tile_start, int32_t total_tiles,
+    uint32_t* __restrict__ hist) {
+  constexpr int T = 256 * PPT;
+  __shared__ uint32_t s_hist[HIST ? 1024 : 1];
+  const int tid = threadIdx.x;
+  const int32_t per = (total_tiles + gridDim.x - 1) / gridDim.x;
+  const int32_t t0 = blockIdx.x * per;
+  const int32_t t1 = min(total_tiles, t0 + per);
+  if (HIST) {
+    for (int d = tid; d < 1024; d += 256) s_hist[d] = 0;
+    __syncthreads();
+  }
+  int32_t cur = -1;
+  auto flush = [&](int32_t seg) {
+    __syncthreads();
+    for (int d = tid; d < 1024; d += 256) {
+      const uint32_t v = s_hist[d];
+      if (v) {
+        atomicAdd(&hist[static_cast<int64_t>(seg) * 1024 + d], v);
+        s_hist[d] = 0;
+      }
+    }
+    __syncthreads();
+  };
+  for (int32_t t = t0; t < t1; ++t) {
+    const TileRef r = find_tile(tile_start, sv.n_seg, t);
+    if (HIST && r.seg != cur && cur >= 0) flush(cur);
+    cur = r.seg;
+    const int32_t sb = seg_begin(sv, r.seg);
+    const int32_t base = sb + r.local * T;
+    const int32_t cnt = min(T, seg_end(sv, r.seg) - base);
+    const float mn = seg_min(sv, r.seg);
+    const float rg = seg_range(sv, r.seg);
+#pragma unroll 4
+    for (int k = 0; k < PPT; ++k) {
+      const int32_t idx = k * 256 + tid;
+      if (idx < cnt) {
+        const int64_t g = static_cast<int64_t>(base) + idx;
+        float x, y, z;
+        if (F3) {
+          x = pts[3 * g + 0];
+          y = pts[3 * g + 1];
+          z = pts[3 * g + 2];
+        } else {
+          const float4 p = reinterpret_cast<const float4*>(pts)[g];
+          x = p.x;
+          y = p.y;
+          z = p.z;
+        }
+        const uint32_t key = encode(x, y, z, mn, rg);
+        keys[g] = key;
+        if (HIST) {
+          atomicAdd(&s_hist[key & 255u], 1u);
+          atomicAdd(&s_hist[256 + ((key >> 8) & 255u)], 1u);
+          atomicAdd(&s_hist[512 + ((key >> 16) & 255u)], 1u);
+          atomicAdd(&s_hist[768 + (key >> 24)], 1u);
+        }
+      }
+    }
+  }
+  if (HIST && cur >= 0) flush(cur);
+}
+
+template <int PPT>
+void launch_ppt(const float* pts, bool f3, uint32_t* keys, const SegView& sv,
+                const int32_t* tile_start, int32_t total_tiles, uint32_t* hist,
+                int grid, hipStream_t s) {
+  if (f3) {
+    if (hist)
+      morton_kernel<PPT, true, true><<<grid, 256, 0, s>>>(pts, keys, sv, tile_start, total_tiles, hist);
+    else
+      morton_kernel<PPT, true, false><<<grid, 256, 0, s>>>(pts, keys, sv, tile_start, total_tiles, hist);
+  } else {
+    if (hist)
+      morton_kernel<PPT, false, true><<<grid, 256, 0, s>>>(pts, keys, sv, tile_start, total_tiles, hist);
+    else
+      morton_kernel<PPT, false, false><<<grid, 256, 0, s>>>(pts, keys, sv, tile_start, total_tiles, hist);
+  }
+}
+
+}  // namespace
+
+bool morton_config_supported(int ppt) {
+  return ppt == 4 || ppt == 8 || ppt == 16 || ppt == 32;
+}
+
+void launch_morton(const float* pts, bool float3, uint32_t* keys,
+                   const SegView& sv, const int32_t* tile_start,
+                   int32_t total_tiles, uint32_t* hist, const TuneTable& tt,
+                   hipStream_t s) {
+  if (total_tiles <= 0) return;
+  const int grid = std::max(1, std::min(total_tiles, tt.morton_grid));
+  switch (tt.morton_ppt) {
+    case 4: launch_ppt<4>(pts, float3, keys, sv, tile_start, total_tiles, hist, grid, s); break;
+    case 8: launch_ppt<8>(pts, float3, keys, sv, tile_start, total_tiles, hist, grid, s); break;
+    case 32: launch_ppt<32>(pts, float3, keys, sv, tile_start, total_tiles, hist, grid, s); break;
+    default: launch_ppt<16>(pts, float3, keys, sv, tile_start, total_tiles, hist, grid, s); break;
+  }
+}
+
+}  // namespace ocp

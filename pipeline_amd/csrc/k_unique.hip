@@ -1,0 +1,142 @@
+// Stage 3: unique compaction of sorted keys (kernels/03_unique.cpp:2-16:
+// keep keys[i] when it differs from keys[i-1]; the reference GPU version,
+// kernels/cuda/03_unique.cu:5-24, runs on one thread).
+//
+// Tiles of 256*IPT keys, ticket-ordered. A key is kept when it is the first
+// of its segment or differs from its predecessor. Per row of 64 keys a
+// ballot gives the keep mask, the lane's rank (mbcnt) and the row count; one
+// wave scans the 4*IPT row counts; one thread does the decoupled lookback of
+// the tile total. Kept keys are compacted in LDS, then written with
+// consecutive threads on consecutive output slots. The segment's last tile
+// writes U and seeds the octree slot count (1 when U >= 1).
+//
+// In place (out == in) is safe: a tile reads all its keys, and its
+// predecessor key, before publishing its count, a tile only writes below
+// its own end, and it can only reach the predecessor key of the next tile
+// when nothing before it was dropped, i.e. with the key that is already
+// there.
+#include "ocp_internal.hpp"
+
+namespace ocp {
+namespace {
+
+template <int IPT>
+__global__ __launch_bounds__(256) void unique_kernel(
+    const uint32_t* in, uint32_t* out, SegView sv,
+    const int32_t* __restrict__ tile_start, uint64_t* __restrict__ flags,
+    uint32_t* __restrict__ ctrl, int32_t* __restrict__ n_unique,
+    int32_t* __restrict__ kalloc) {
+  constexpr int THREADS = 256;
+  constexpr int NW = THREADS / 64;
+  constexpr int T = THREADS * IPT;
+  static_assert(IPT * NW <= 64, "row counts are scanned by one wave");
+  __shared__ uint32_t s_keys[T];
+  __shared__ uint32_t s_rows[IPT * NW];
+  __shared__ int32_t s_ticket;
+  __shared__ uint32_t s_excl;
+  __shared__ uint32_t s_total;
+
+  const int tid = threadIdx.x;
+  const uint32_t w = wave_id();
+  const uint32_t lane = lane_id();
+  if (tid == 0) s_ticket = static_cast<int32_t>(atomicAdd(&ctrl[kCtrlTicket0 + kStageUnique], 1u));
+  __syncthreads();
+  const int32_t t = s_ticket;
+  const uint32_t tag = ctrl[kCtrlEpoch] * 8u + kStageUnique;
+  const TileRef r = find_tile(tile_start, sv.n_seg, t);
+  const int32_t sb = seg_begin(sv, r.seg);
+  const int32_t se = seg_end(sv, r.seg);
+  const int32_t base = sb + r.local * T;
+  const int32_t cnt = min(T, se - base);
+
+  uint32_t keys[IPT];
+#pragma unroll
+  for (int k = 0; k < IPT; ++k) {
+    const int32_t idx = k * THREADS + tid;
+    keys[k] = idx < cnt ? in[static_cast<int64_t>(base) + idx] : 0u;
+    s_keys[idx] = keys[k];
+  }
+  const uint32_t before = (r.local > 0) ? in[static_cast<int64_t>(base) - 1] : 0u;
+  __syncthreads();
+
+  uint64_t keep_mask[IPT];
+#pragma unroll
+  for (int k = 0; k < IPT; ++k) {
+    const int32_t idx = k * THREADS + tid;
+    bool keep = false;
+    if (idx < cnt) {
+      if (idx == 0)
+        keep = (r.local == 0) || keys[k] != before;
+      else
+        keep = keys[k] != s_keys[idx - 1];
+    }
+    keep_mask[k] = __ballot(keep);
+    if (lane == 0) s_rows[k * NW + w] = static_cast<uint32_t>(__popcll(keep_mask[k]));
+  }
+  __syncthreads();
+  if (w == 0) {
+    const uint32_t v = lane < IPT * NW ? s_rows[lane] : 0u;
+    const uint32_t incl = wave_incl_scan(v);
+    if (lane < IPT * NW) s_rows[lane] = incl - v;
+    const uint32_t total = __shfl(incl, IPT * NW - 1, 64);
+    if (lane == 0) {
+      uint32_t prev = 0;
+      if (r.local == 0) {
+        flag_store(&flags[t], flag_pack(tag, kFlagInc, total));
+      } else {
+        flag_store(&flags[t], flag_pack(tag, kFlagAgg, total));
+        prev = lookback(flags, 1, t, tag);
+        flag_store(&flags[t], flag_pack(tag, kFlagInc, prev + total));
+      }
+      s_excl = prev;
+      s_total = total;
+      const int32_t n_tiles = (se - sb + T - 1) / T;
+      if (r.local == n_tiles - 1) {
+        const int32_t u = static_cast<int32_t>(prev + total);
+        n_unique[r.seg] = u;
+        if (kalloc) kalloc[r.seg] = u >= 1 ? 1 : 0;
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t excl = s_excl;
+  const uint32_t total = s_total;
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int k = 0; k < IPT; ++k) {
+    if ((keep_mask[k] >> lane) & 1ull) {
+      const uint32_t pos = s_rows[k * NW + w] + static_cast<uint32_t>(__popcll(keep_mask[k] & lt));
+      s_keys[pos] = keys[k];
+    }
+  }
+  __syncthreads();
+  uint32_t* dst = out + sb + excl;
+  for (uint32_t p = tid; p < total; p += THREADS) dst[p] = s_keys[p];
+}
+
+template <int IPT>
+void launch_ipt(const uint32_t* in, uint32_t* out, const SegView& sv,
+                const int32_t* tile_start, int32_t total_tiles,
+                uint64_t* flags, uint32_t* ctrl, int32_t* n_unique,
+                int32_t* kalloc, hipStream_t s) {
+  unique_kernel<IPT><<<total_tiles, 256, 0, s>>>(in, out, sv, tile_start,
+                                                 flags, ctrl, n_unique, kalloc);
+}
+
+}  // namespace
+
+bool scan_config_supported(int ipt) { return ipt == 4 || ipt == 8 || ipt == 16; }
+
+void launch_unique(const uint32_t* in, uint32_t* out, const SegView& sv,
+                   const int32_t* tile_start, int32_t total_tiles,
+                   uint64_t* flags, uint32_t* ctrl, int32_t* n_unique,
+                   int32_t* kalloc, const TuneTable& tt, hipStream_t s) {
+  if (total_tiles <= 0) return;
+  switch (tt.scan_ipt) {
+    case 4: launch_ipt<4>(in, out, sv, tile_start, total_tiles, flags, ctrl, n_unique, kalloc, s); break;
+    case 8: launch_ipt<8>(in, out, sv, tile_start, total_tiles, flags, ctrl, n_unique, kalloc, s); break;
+    default: launch_ipt<16>(in, out, sv, tile_start, total_tiles, flags, ctrl, n_unique, kalloc, s); break;
+  }
+}
+
+}  // namespace ocp

@@ -1,0 +1,206 @@
+// Device-side building blocks shared by every stage kernel (gfx950, wave64).
+//
+// - morton codec: the reference's 10-bit-per-axis encode and decode
+//   (include/shared/morton.h:15-80), integer-exact, float parts compiled with
+//   -ffp-contract=off so products and sums round separately;
+// - wave64 scans on ds_swizzle/bpermute, block scans over LDS;
+// - decoupled-lookback flags: 64-bit words {tag:30, status:2, count:32}
+//   published and polled with agent-scope atomics so that they are coherent
+//   across the eight XCD-private L2s; the tag carries an epoch so that stale
+//   flags from an earlier launch are never mistaken for ready ones and the
+//   flag array never needs clearing;
+// - the segment view: every stage kernel works on a batch of independent
+//   clouds described by a point-offset table (one cloud = one segment); a
+//   tile never straddles two segments.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ocp {
+
+constexpr int kWave = 64;
+constexpr int kMortonBits = 30;
+
+// ---------------------------------------------------------------- lanes
+__device__ __forceinline__ uint32_t lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const uint32_t l = lane_id();
+  return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+__device__ __forceinline__ uint32_t wave_id() { return threadIdx.x >> 6; }
+
+// ---------------------------------------------------------------- morton
+__device__ __forceinline__ uint32_t split_by_3(uint32_t a) {
+  uint32_t x = a & 0x3ffu;
+  x = (x | x << 16) & 0x30000ffu;
+  x = (x | x << 8) & 0x0300f00fu;
+  x = (x | x << 4) & 0x30c30c3u;
+  x = (x | x << 2) & 0x9249249u;
+  return x;
+}
+__device__ __forceinline__ uint32_t third_bits(uint32_t m) {
+  uint32_t x = m & 0x9249249u;
+  x = (x ^ (x >> 2)) & 0x30c30c3u;
+  x = (x ^ (x >> 4)) & 0x0300f00fu;
+  x = (x ^ (x >> 8)) & 0x30000ffu;
+  x = (x ^ (x >> 16)) & 0x3ffu;
+  return x;
+}
+// (a - min) / range * 1024, truncated toward zero: v_sub, the correctly
+// rounded v_div sequence, v_mul, v_cvt_i32. Going through int32 gives the
+// same bits as the x86 truncating conversion for coordinates a little below
+// min as well (the reference leaves those undefined).
+__device__ __forceinline__ uint32_t quantize(float a, float mn, float rg) {
+  const float v = (a - mn) / rg * 1024.0f;
+  return static_cast<uint32_t>(static_cast<int32_t>(v));
+}
+__device__ __forceinline__ uint32_t encode(float x, float y, float z, float mn,
+                                           float rg) {
+  return split_by_3(quantize(x, mn, rg)) |
+         (split_by_3(quantize(y, mn, rg)) << 1) |
+         (split_by_3(quantize(z, mn, rg)) << 2);
+}
+__device__ __forceinline__ float decode_axis(uint32_t code, int axis, float mn,
+                                             float rg) {
+  const float t = static_cast<float>(third_bits(code >> axis)) / 1024.0f;
+  return t * rg + mn;
+}
+
+// ---------------------------------------------------------------- scans
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const uint32_t l = lane_id();
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t t = __shfl_up(v, o, kWave);
+    if (l >= static_cast<uint32_t>(o)) v += t;
+  }
+  return v;
+}
+__device__ __forceinline__ int32_t wave_incl_max(int32_t v) {
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) v = max(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+
+// Exclusive scan of one value per thread over a block of THREADS threads.
+// `tmp` holds THREADS/64 words of LDS. Ends with a barrier so `tmp` can be
+// reused right away.
+template <int THREADS>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* tmp,
+                                                    uint32_t* total) {
+  constexpr int NW = THREADS / kWave;
+  const uint32_t incl = wave_incl_scan(v);
+  const uint32_t w = wave_id();
+  if (lane_id() == kWave - 1) tmp[w] = incl;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < NW; ++k) {
+    const uint32_t t = tmp[k];
+    off += (static_cast<uint32_t>(k) < w) ? t : 0u;
+    tot += t;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + incl - v;
+}
+
+// ---------------------------------------------------------------- lookback
+constexpr uint32_t kFlagAgg = 1;  // the tile's own count is published
+constexpr uint32_t kFlagInc = 2;  // the inclusive prefix is published
+
+__device__ __forceinline__ uint64_t flag_pack(uint32_t tag, uint32_t status,
+                                              uint32_t count) {
+  return (static_cast<uint64_t>(((tag & 0x3fffffffu) << 2) | status) << 32) |
+         count;
+}
+__device__ __forceinline__ void flag_store(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t flag_load(const uint64_t* p) {
+  return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+// Sum of the counts of tiles (first, t) of one chain, polling tile k-1,
+// k-2, ... until an inclusive prefix is found. Tiles are handed out by a
+// ticket counter, so every earlier tile of the chain is already resident
+// and will publish; the wait always ends.
+__device__ __forceinline__ uint32_t lookback(const uint64_t* flags,
+                                             uint32_t stride, int32_t t,
+                                             uint32_t tag) {
+  uint32_t excl = 0;
+  int32_t k = t - 1;
+  const uint32_t want = tag & 0x3fffffffu;
+  while (true) {
+    const uint64_t v = flag_load(flags + static_cast<int64_t>(k) * stride);
+    const uint32_t hi = static_cast<uint32_t>(v >> 32);
+    const uint32_t st = hi & 3u;
+    if ((hi >> 2) != want || st == 0) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    excl += static_cast<uint32_t>(v);
+    if (st == kFlagInc) break;
+    --k;
+  }
+  return excl;
+}
+
+// ---------------------------------------------------------------- segments
+// A batch of independent clouds. `start` holds n_seg+1 point offsets; with
+// start == nullptr the batch is the single segment [0, n1) with box
+// (min1, range1). Per-segment boxes come from mn/rg when non-null.
+struct SegView {
+  int32_t n_seg;
+  const int32_t* start;
+  const float* mn;
+  const float* rg;
+  int32_t n1;
+  float min1;
+  float range1;
+};
+
+__device__ __forceinline__ int32_t seg_begin(const SegView& s, int32_t k) {
+  return s.start ? s.start[k] : 0;
+}
+__device__ __forceinline__ int32_t seg_end(const SegView& s, int32_t k) {
+  return s.start ? s.start[k + 1] : s.n1;
+}
+__device__ __forceinline__ float seg_min(const SegView& s, int32_t k) {
+  return s.mn ? s.mn[k] : s.min1;
+}
+__device__ __forceinline__ float seg_range(const SegView& s, int32_t k) {
+  return s.rg ? s.rg[k] : s.range1;
+}
+
+// Tile map for one tile size: tile_start[k] = first global tile of segment
+// k (n_seg+1 entries, on device). nullptr means a single segment.
+struct TileRef {
+  int32_t seg;
+  int32_t local;  // tile index inside its segment
+};
+__device__ __forceinline__ TileRef find_tile(const int32_t* tile_start,
+                                             int32_t n_seg, int32_t t) {
+  if (!tile_start) return {0, t};
+  int32_t lo = 0, hi = n_seg;  // invariant: tile_start[lo] <= t < tile_start[hi]
+  while (hi - lo > 1) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (tile_start[mid] <= t) lo = mid; else hi = mid;
+  }
+  return {lo, t - tile_start[lo]};
+}
+
+// Reference OctNode layout (include/shared/types.h:28-34), 60 bytes.
+struct OctNode {
+  int32_t children[8];
+  float corner[4];
+  float cell_size;
+  int32_t child_node_mask;
+  int32_t child_leaf_mask;
+};
+static_assert(sizeof(OctNode) == 60, "OctNode must stay 60 bytes");
+
+}  // namespace ocp

@@ -1,0 +1,135 @@
+// Host-side launchers of the stage kernels (one translation unit per stage)
+// and the shared control block. Every launcher is asynchronous on `stream`
+// and takes device pointers only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ocp_device.hpp"
+
+namespace ocp {
+
+// Control words in device memory (one block per context).
+enum CtrlSlot : int {
+  kCtrlTicket0 = 0,    // 8 tile-ticket counters, one per lookback stage
+  kCtrlEpoch = 8,      // bumped once per launch group; tags lookback flags
+  kCtrlOctTotal = 9,   // total octree slots of the batch
+  kCtrlOctOk = 10,     // 1 when the octree fits the node capacity
+  kCtrlLearnFail = 11, // learned sort: buckets that overflowed
+  kCtrlLearnBad = 12,  // learned sort: misordered bucket boundaries
+  kCtrlWords = 64,
+};
+
+// Lookback stage ids (ticket counter and flag tag).
+enum Stage : uint32_t {
+  kStagePass0 = 0,  // OneSweep passes 0..3
+  kStageUnique = 4,
+  kStageScan = 5,
+  kStageBucket = 6,
+};
+
+// Kernel configuration chosen by the autotuner (or the defaults).
+struct TuneTable {
+  int sort_waves = 8;      // OneSweep workgroup = sort_waves * 64 threads
+  int sort_kpt = 15;       // keys per thread in OneSweep
+  int morton_ppt = 16;     // points per thread in the Morton kernel
+  int morton_grid = 1024;  // persistent Morton workgroups
+  int scan_ipt = 16;       // items per thread in unique / edge-scan tiles
+};
+
+inline int sort_tile(const TuneTable& t) { return t.sort_waves * 64 * t.sort_kpt; }
+inline int morton_tile(const TuneTable& t) { return 256 * t.morton_ppt; }
+inline int scan_tile(const TuneTable& t) { return 256 * t.scan_ipt; }
+constexpr int kNodeTile = 256;  // one thread per BRT node
+
+bool sort_config_supported(int waves, int kpt);
+bool scan_config_supported(int ipt);
+bool morton_config_supported(int ppt);
+
+// ---- utility kernels (k_util.hip)
+void launch_fill_u32(uint32_t* p, uint32_t value, int64_t n_words,
+                     hipStream_t s);
+// Zero tickets, bump epoch; optionally zero n_hist words of histogram.
+void launch_begin(uint32_t* ctrl, uint32_t* hist, int64_t n_hist,
+                  hipStream_t s);
+void launch_oct_offsets(const int32_t* kalloc, int32_t n_seg,
+                        int32_t* oct_start, uint32_t* ctrl, int64_t k_cap,
+                        hipStream_t s);
+void launch_zero_nodes(OctNode* nodes, const uint32_t* ctrl, int64_t k_cap,
+                       hipStream_t s);
+
+// ---- stage 1 (k_morton.hip)
+void launch_morton(const float* pts, bool float3, uint32_t* keys,
+                   const SegView& sv, const int32_t* tile_start,
+                   int32_t total_tiles, uint32_t* hist, const TuneTable& tt,
+                   hipStream_t s);
+
+// ---- stage 2 (k_onesweep.hip)
+void launch_histogram(const uint32_t* keys, const SegView& sv,
+                      const int32_t* tile_start, int32_t total_tiles,
+                      uint32_t* hist, const TuneTable& tt, hipStream_t s);
+void launch_onesweep_pass(const uint32_t* in, uint32_t* out,
+                          const SegView& sv, const int32_t* tile_start,
+                          int32_t total_tiles, const uint32_t* hist,
+                          uint64_t* flags, uint32_t* ctrl, int pass,
+                          const TuneTable& tt, hipStream_t s);
+
+// ---- stage 3 (k_unique.hip)
+void launch_unique(const uint32_t* in, uint32_t* out, const SegView& sv,
+                   const int32_t* tile_start, int32_t total_tiles,
+                   uint64_t* flags, uint32_t* ctrl, int32_t* n_unique,
+                   int32_t* kalloc, const TuneTable& tt, hipStream_t s);
+
+// ---- stage 4 (k_radix_tree.hip)
+// n_brt: per-segment node counts; when n_unique != nullptr the count of
+// segment k is n_unique[k]-1, else n_brt_host (single segment).
+void launch_radix_tree(const uint32_t* codes, uint8_t* prefix_n,
+                       uint8_t* has_leaf_left, uint8_t* has_leaf_right,
+                       int32_t* left_child, int32_t* parent, const SegView& sv,
+                       const int32_t* tile_start, int32_t total_tiles,
+                       const int32_t* n_unique, int32_t n_brt_host,
+                       hipStream_t s);
+
+// ---- stages 5 + 6 (k_edge_scan.hip)
+// mode 0: edge count + inclusive scan fused; 1: edge count only;
+// 2: inclusive scan of `edge` only.
+void launch_edge_scan(const uint8_t* prefix_n, const int32_t* parent,
+                      int32_t* edge, int32_t* offsets, const SegView& sv,
+                      const int32_t* tile_start, int32_t total_tiles,
+                      const int32_t* n_unique, int32_t n_brt_host,
+                      uint64_t* flags, uint32_t* ctrl, int32_t* kalloc,
+                      int mode, const TuneTable& tt, hipStream_t s);
+
+// ---- stage 7 (k_octree.hip)
+struct OctArgs {
+  OctNode* nodes;            // base of all segments' nodes
+  const int32_t* oct_start;  // per-segment node base (nullptr: 0)
+  const int32_t* kalloc;     // per-segment slot count (nullptr: k_host)
+  int32_t k_host;
+  const uint32_t* ctrl;      // kCtrlOctOk checked when non-null
+  const uint32_t* codes;
+  const uint8_t* prefix_n;
+  const uint8_t* has_leaf_left;
+  const uint8_t* has_leaf_right;
+  const int32_t* left_child;
+  const int32_t* parent;
+  const int32_t* edge;
+  const int32_t* offsets;
+  const int32_t* n_unique;  // per segment (nullptr: n_brt_host + 1)
+  int32_t n_brt_host;
+};
+void launch_oct_root(OctNode* nodes, const uint32_t* codes,
+                     const uint8_t* prefix_n, int32_t n_brt, float mn,
+                     float rg, hipStream_t s);
+// phase bits: 1 = make writes, 2 = link writes, 4 = leaf-mask AND-NOT
+void launch_oct_scatter(const OctArgs& a, const SegView& sv,
+                        const int32_t* tile_start, int32_t total_tiles,
+                        int phases, hipStream_t s);
+// resolve bits: 1 = resolve make keys, 2 = resolve link keys,
+//               4 = write corners and cell sizes, 8 = write the root node
+void launch_oct_finalize(const OctArgs& a, const SegView& sv,
+                         const int32_t* tile_start, int32_t total_tiles,
+                         int resolve, hipStream_t s);
+
+}  // namespace ocp

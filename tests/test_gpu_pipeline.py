@@ -1,0 +1,103 @@
+"""GPU parity of the fused device-resident build (plain, graph replay,
+batched) against the oracle's full chain, stage output by stage output."""
+import numpy as np
+import pytest
+
+import oracle
+from pipeline_amd import clouds
+
+pytestmark = pytest.mark.gpu
+
+STAGES = ["prefix_n", "has_leaf_left", "has_leaf_right", "left_child", "parent",
+          "edge_count", "offsets"]
+
+
+def _check_segment(out, s, exp):
+    np.testing.assert_array_equal(out.sorted_keys[s], exp["sorted"], err_msg="sorted")
+    assert int(out.n_unique[s]) == exp["n_unique"]
+    np.testing.assert_array_equal(out.unique_keys[s], exp["unique"], err_msg="unique")
+    for k in STAGES:
+        np.testing.assert_array_equal(getattr(out, k)[s], exp[k], err_msg=k)
+    assert int(out.n_oct[s]) == exp["n_oct"]
+    np.testing.assert_array_equal(out.nodes[s].view(np.uint8), exp["nodes"].view(np.uint8),
+                                  err_msg="nodes")
+
+
+def _plain(ctx, pts, mn=0.0, rg=1024.0):
+    d = ctx.upload(pts)
+    ctx.build(d, pts.shape[0], mn, rg)
+    out = ctx.fetch()
+    d.free()
+    return out
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 4096, 7681, 307200])
+def test_plain_uniform(ctx, n):
+    pts = clouds.uniform(n, seed=114514 + n)
+    out = _plain(ctx, pts)
+    _check_segment(out, 0, oracle.build_plain(pts))
+
+
+@pytest.mark.parametrize("n", [5000, 262144])
+def test_plain_clustered(ctx, n):
+    pts = clouds.clustered(n, seed=7 + n)
+    out = _plain(ctx, pts)
+    _check_segment(out, 0, oracle.build_plain(pts))
+
+
+def test_plain_all_duplicates_and_float3(ctx):
+    pts = np.tile(np.array([[1.5, 2.5, 3.5, 0.0]], np.float32), (10000, 1))
+    _check_segment(_plain(ctx, pts), 0, oracle.build_plain(pts))
+    p3 = clouds.uniform(70001, seed=3, float4=False)
+    _check_segment(_plain(ctx, p3), 0, oracle.build_plain(p3))
+    pts = clouds.clustered(30000, seed=4)
+    pts[:, :3] = pts[:, :3] * np.float32(0.25) + np.float32(-40.0)
+    _check_segment(_plain(ctx, pts, -40.0, 256.0), 0, oracle.build_plain(pts, -40.0, 256.0))
+
+
+def test_graph_replay_matches_eager(ctx):
+    pts = clouds.clustered(200000, seed=99)
+    exp = oracle.build_plain(pts)
+    d = ctx.upload(pts)
+    ctx.set_graph_mode(True)
+    try:
+        for _ in range(3):
+            ctx.build(d, pts.shape[0], 0.0, 1024.0)
+            ctx.synchronize()
+            _check_segment(ctx.fetch(), 0, exp)
+        # new data in the same buffer: the replay must see it
+        pts2 = clouds.uniform(200000, seed=100)
+        d.copy_from(pts2)
+        ctx.build(d, pts2.shape[0], 0.0, 1024.0)
+        _check_segment(ctx.fetch(), 0, oracle.build_plain(pts2))
+    finally:
+        ctx.set_graph_mode(False)
+        d.free()
+
+
+def _batched_case(n_clouds, lo, hi, seed):
+    return clouds.batch(n_clouds, lo, hi, seed=seed)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batched_matches_per_cloud_oracle(ctx, graph):
+    pts, offs, smin, srg, _ = _batched_case(24, 1, 60000, seed=5)
+    # include empty and single-point clouds
+    offs = np.concatenate([[0, 0], offs[1:] + 0]).astype(np.int32)
+    smin = np.concatenate([[0.0], smin]).astype(np.float32)
+    srg = np.concatenate([[1.0], srg]).astype(np.float32)
+    d = ctx.upload(pts)
+    ctx.set_graph_mode(graph)
+    try:
+        for rep in range(2 if graph else 1):
+            ctx.build_batched(d, offs, smin, srg)
+            out = ctx.fetch(offs)
+            for s in range(offs.size - 1):
+                seg = pts[offs[s]:offs[s + 1]]
+                if seg.shape[0] == 0:
+                    assert out.n_unique[s] == 0 and out.n_oct[s] == 0
+                    continue
+                _check_segment(out, s, oracle.build_plain(seg, float(smin[s]), float(srg[s])))
+    finally:
+        ctx.set_graph_mode(False)
+        d.free()
