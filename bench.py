@@ -1,0 +1,186 @@
+"""Benchmark of the octree-construction pipeline on one MI355X.
+
+Default workload: the plain pipeline on 10,000,000 float4 points uniform in
+[0, 1024)^3 (seed 114514), 32-bit Morton codes, all seven stages on device,
+replayed from a captured hipGraph with the input already resident in HBM.
+
+One step = one whole build (Morton -> OneSweep -> unique -> radix tree ->
+edge count + scan -> octree). ``value`` is N / (time per build) in Mpoints/s.
+
+Also reported (same JSON line):
+  roofline      the dominant kernel's algorithmic bytes per launch / its
+                average duration from HIP events on the context stream;
+  e2e_roofline  the whole build's algorithmic bytes (60N + 61U + 60K) / the
+                time per build, against 8 TB/s;
+  kernels_ms    every kernel's average duration in eager builds;
+  cpu_baseline  the CPU oracle (the reference CPU path restated, OpenMP) timed
+                on this host in the same run.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload plain|clustered]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "end-to-end octree Mpoints/s + HBM roofline %, plain vs batched/learned-sort"
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(name: str, n: int, u: int, k: int, float4: bool = True) -> float:
+    """Algorithmic HBM bytes of one launch (per-unit figure x units)."""
+    m = max(u - 1, 0)
+    pt = 16 if float4 else 12
+    table = {
+        "morton_hist": (pt + 4) * n,            # point read + key write
+        "onesweep_pass0": 8 * n,                # key read + key write
+        "onesweep_pass1": 8 * n,
+        "onesweep_pass2": 8 * n,
+        "onesweep_pass3": 8 * n,
+        "unique": 4 * n + 4 * u,                # sorted read + unique write
+        "radix_tree": 4 * u + 11 * m,           # codes read; 1+1+1+4+4 written
+        "edge_scan": 14 * m,                    # 1+4+1 read; 4+4 written
+        "octree": 24 * m + 60 * k,              # BRT reads + one 60-B node per slot
+    }
+    return float(table.get(name, 0.0))
+
+
+def e2e_bytes(n: int, u: int, k: int) -> float:
+    # 20N (Morton) + 36N (sort) + 4N+4U (unique) + 4U+11M (tree) + 10M (edge)
+    # + 8M (scan) + 24M+60K (octree), M = U-1.
+    return 60.0 * n + 61.0 * u + 60.0 * k
+
+
+def omp_threads() -> int:
+    v = os.environ.get("OMP_NUM_THREADS")
+    if v and v.isdigit() and int(v) > 0:
+        return int(v)
+    return min(os.cpu_count() or 1, 16)
+
+
+def cpu_baseline(pts: np.ndarray, budget_s: float = 20.0):
+    """The CPU oracle's full chain on this host (OpenMP), median of runs."""
+    import oracle
+
+    nt = omp_threads()
+    t0 = time.perf_counter()
+    ms1, u, k = oracle.time_pipeline(pts, 0.0, 1024.0, nthreads=nt, reps=1)
+    one = time.perf_counter() - t0  # warm-up + one timed run
+    reps = int(max(1, min(5, (budget_s - one) / max(one / 2, 1e-3))))
+    ms, u, k = oracle.time_pipeline(pts, 0.0, 1024.0, nthreads=nt, reps=reps)
+    n = pts.shape[0]
+    return {
+        "value": n / (ms["total"] * 1e-3) / 1e6,
+        "unit": "Mpoints/s",
+        "cores": nt,
+        "kind": "port",
+        "sample": (f"full workload ({n} points), median of {reps} timed runs after "
+                   f"a warm-up, OpenMP on {nt} threads"),
+        "stage_ms": {kk: round(v, 3) for kk, v in ms.items()},
+        "n_unique": u,
+        "n_oct": k,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="plain", choices=["plain", "clustered"])
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the pipeline has no multi-GPU path in this build)", file=sys.stderr)
+        return 2
+
+    import pipeline_amd
+    from pipeline_amd import clouds
+
+    if args.workload == "plain":
+        n = args.n or 10_000_000
+        pts = clouds.uniform(n, seed=114514)
+        wl = f"plain, {n} float4 points uniform in [0,1024)^3, 32-bit Morton"
+    else:
+        n = args.n or (1 << 24)
+        pts = clouds.clustered(n, seed=114514)
+        wl = f"plain, {n} float4 points clustered (plane/cylinders/gaussians), 32-bit Morton"
+
+    ctx = pipeline_amd.Context(0)
+    d = ctx.upload(pts)
+    # Eager builds with per-kernel HIP events.
+    ctx.set_stage_timing(True)
+    kms: dict[str, list[float]] = {}
+    for _ in range(4):
+        ctx.build(d, n, 0.0, 1024.0)
+        ctx.synchronize()
+        for kname, v in ctx.kernel_ms().items():
+            kms.setdefault(kname, []).append(v)
+    ctx.set_stage_timing(False)
+    kmean = {kname: float(np.mean(v[1:])) for kname, v in kms.items()}
+    u_arr, k_arr = ctx.counts(1)
+    u, k = int(u_arr[0]), int(k_arr[0])
+
+    # Timed region: K graph replays, synchronised on both sides.
+    ctx.set_graph_mode(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ms_per = ctx.time_builds(d, args.warmup, args.steps, n=n)
+    wall = time.perf_counter() - t0
+    ctx.synchronize()
+    u_arr, k_arr = ctx.counts(1)
+    assert int(u_arr[0]) == u and int(k_arr[0]) == k
+
+    value = n / (ms_per * 1e-3) / 1e6
+    dom = max((kk for kk in kmean if kernel_bytes(kk, n, u, k) > 0), key=lambda kk: kmean[kk])
+    dom_bytes = kernel_bytes(dom, n, u, k)
+    achieved = dom_bytes / (kmean[dom] * 1e-3) / 1e9
+    eb = e2e_bytes(n, u, k)
+    e2e = eb / (ms_per * 1e-3) / 1e9
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mpoints/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": wl, "n_points": n, "n_unique": u, "n_oct": k,
+                   "seed": 114514, "hipgraph": True, "tune": ctx.get_tune()},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": dom_bytes, "avg_ms": round(kmean[dom], 5)},
+        "e2e_roofline": {"achieved": round(e2e, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(e2e / HBM_PEAK_GBS, 4), "bytes": eb},
+        "kernels_ms": {kk: round(v, 5) for kk, v in kmean.items()},
+        "wall_s_timed": round(wall, 3),
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(pts)
+        cb_ok = cb["n_unique"] == u and cb["n_oct"] == k
+        line["cpu_baseline"] = cb
+        line["cpu_counts_match"] = bool(cb_ok)
+    ctx.close()
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -170,6 +170,11 @@ int ocp_build(ocp_ctx ctx, const float* points, int float3, int64_t n,
 int ocp_build_batched(ocp_ctx ctx, const float* points, int float3,
                       const int32_t* seg_offsets, const float* seg_min,
                       const float* seg_range, int32_t n_seg);
+/* The checks ocp_build_batched applies to its table, without a context or
+ * device: offsets start at 0 and do not decrease, each segment holds at most
+ * 2^27 points, the batch at most 2^30-1, ranges are > 0. */
+int ocp_check_batch(const int32_t* seg_offsets, const float* seg_min,
+                    const float* seg_range, int32_t n_seg);
 /* Capture each distinct build shape once into a hipGraph and replay it. */
 int ocp_set_graph_mode(ocp_ctx ctx, int enable);
 
@@ -201,6 +206,12 @@ int ocp_get_counts(ocp_ctx ctx, int32_t* n_unique, int32_t* n_oct);
  * unique, radix_tree, edge_scan, octree, total]. */
 int ocp_set_stage_timing(ocp_ctx ctx, int enable);
 int ocp_get_stage_ms(ocp_ctx ctx, float* ms7);
+/* Per-kernel device time of the last eager build with stage timing on, in
+ * launch order, each bracketed by HIP events on the context stream. Writes
+ * up to max_kernels entries (names are static strings) and the number of
+ * kernels a build launches to n_kernels. */
+int ocp_get_kernel_ms(ocp_ctx ctx, float* ms, const char** names,
+                      int32_t max_kernels, int32_t* n_kernels);
 /* Time `steps` builds (after `warmup`) of the configured kind on the
  * context stream with HIP events, bracketed by device synchronisation.
  * Writes the mean ms per build. */

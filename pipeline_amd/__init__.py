@@ -93,11 +93,13 @@ SIGNATURES = {
     "ocp_k_LinkLeafNodes": (_INT, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32]),
     "ocp_build": (_INT, [_P, _P, _INT, _I64, _F, _F]),
     "ocp_build_batched": (_INT, [_P, _P, _INT, _P, _P, _P, _I32]),
+    "ocp_check_batch": (_INT, [_P, _P, _P, _I32]),
     "ocp_set_graph_mode": (_INT, [_P, _INT]),
     "ocp_get_results": (_INT, [_P, ctypes.POINTER(Results)]),
     "ocp_get_counts": (_INT, [_P, _P, _P]),
     "ocp_set_stage_timing": (_INT, [_P, _INT]),
     "ocp_get_stage_ms": (_INT, [_P, _P]),
+    "ocp_get_kernel_ms": (_INT, [_P, _P, _P, _I32, ctypes.POINTER(_I32)]),
     "ocp_time_builds": (_INT, [_P, _P, _INT, _P, _P, _P, _I32, _INT, _INT, ctypes.POINTER(_F)]),
     "ocp_get_tune": (_INT, [_P, ctypes.POINTER(Tune)]),
     "ocp_set_tune": (_INT, [_P, ctypes.POINTER(Tune)]),
@@ -127,6 +129,14 @@ def load_library(path: str = LIB_PATH):
 
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def check_batch(seg_offsets, seg_min, seg_range) -> int:
+    """Status the batched build would return for this table (no GPU used)."""
+    offs = np.ascontiguousarray(seg_offsets, np.int32)
+    mn = np.ascontiguousarray(seg_min, np.float32)
+    rg = np.ascontiguousarray(seg_range, np.float32)
+    return int(load_library().ocp_check_batch(_ptr(offs), _ptr(mn), _ptr(rg), offs.size - 1))
 
 
 class DeviceArray:
@@ -324,6 +334,14 @@ class Context:
         self._check(self.lib.ocp_get_stage_ms(self.h, _ptr(ms)))
         names = ["morton", "sort", "unique", "radix_tree", "edge_scan", "octree", "total"]
         return dict(zip(names, ms.astype(float).tolist()))
+
+    def kernel_ms(self) -> dict:
+        """Per-kernel ms of the last eager build (stage timing on)."""
+        n = ctypes.c_int32()
+        ms = np.zeros(64, np.float32)
+        names = (ctypes.c_char_p * 64)()
+        self._check(self.lib.ocp_get_kernel_ms(self.h, _ptr(ms), names, 64, ctypes.byref(n)))
+        return {names[i].decode(): float(ms[i]) for i in range(n.value)}
 
     def build(self, points: DeviceArray, n: int | None = None, min_coord=0.0,
               range_=1024.0):

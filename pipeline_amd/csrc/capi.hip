@@ -62,6 +62,16 @@ struct Plan {
   const int32_t* ts_node = nullptr;
 };
 
+// Kernels of one build, in launch order (per-kernel event timing).
+enum KernelId : int {
+  kKBegin = 0, kKMorton, kKPass0, kKPass1, kKPass2, kKPass3, kKFill, kKUnique,
+  kKRadixTree, kKEdgeScan, kKOctOffsets, kKOctBuild, kKernelCount
+};
+const char* const kKernelNames[kKernelCount] = {
+    "begin", "morton_hist", "onesweep_pass0", "onesweep_pass1", "onesweep_pass2",
+    "onesweep_pass3", "fill_counts", "unique", "radix_tree", "edge_scan",
+    "oct_offsets", "octree"};
+
 struct LastBuild {
   bool valid = false;
   bool batched = false;
@@ -97,6 +107,7 @@ struct ocp_context {
   std::map<std::string, hipGraphExec_t> graphs;
   bool stage_timing = false;
   hipEvent_t ev[8] = {};
+  hipEvent_t kev[2 * kKernelCount] = {};
   bool ev_recorded = false;
   LastBuild last;
 };
@@ -251,19 +262,29 @@ Plan plain_plan(const TuneTable& tt, int64_t n, float mn, float rg) {
 
 // Batched plan: validates the table, uploads offsets, boxes and tile maps
 // (only when they changed).
-int batched_plan(ocp_ctx c, const int32_t* offsets, const float* smin,
-                 const float* srg, int32_t n_seg, Plan* out) {
-  if (n_seg < 1) return fail(c, OCP_ERR_INVALID, "n_seg must be >= 1");
-  if (!offsets || !smin || !srg) return fail(c, OCP_ERR_INVALID, "null segment table");
-  if (offsets[0] != 0) return fail(c, OCP_ERR_INVALID, "seg_offsets[0] must be 0");
+// Host-side checks of a batch table; no device state is touched.
+int validate_batch(const int32_t* offsets, const float* smin, const float* srg,
+                   int32_t n_seg, std::string* why) {
+  if (n_seg < 1) return *why = "n_seg must be >= 1", OCP_ERR_INVALID;
+  if (!offsets || !smin || !srg) return *why = "null segment table", OCP_ERR_INVALID;
+  if (offsets[0] != 0) return *why = "seg_offsets[0] must be 0", OCP_ERR_INVALID;
   for (int32_t k = 0; k < n_seg; ++k) {
     const int64_t len = int64_t(offsets[k + 1]) - offsets[k];
-    if (len < 0) return fail(c, OCP_ERR_INVALID, "seg_offsets must be non-decreasing");
-    if (len > kMaxSegPoints) return fail(c, OCP_ERR_CAPACITY, "segment larger than 2^27 points");
-    if (!(srg[k] > 0.0f)) return fail(c, OCP_ERR_INVALID, "seg_range must be > 0");
+    if (len < 0) return *why = "seg_offsets must be non-decreasing", OCP_ERR_INVALID;
+    if (len > kMaxSegPoints) return *why = "segment larger than 2^27 points", OCP_ERR_CAPACITY;
+    if (!(srg[k] > 0.0f)) return *why = "seg_range must be > 0", OCP_ERR_INVALID;
+    if (!(smin[k] == smin[k])) return *why = "seg_min must not be NaN", OCP_ERR_INVALID;
   }
+  if (int64_t(offsets[n_seg]) > kMaxPoints)
+    return *why = "more than 2^30-1 points", OCP_ERR_CAPACITY;
+  return OCP_OK;
+}
+
+int batched_plan(ocp_ctx c, const int32_t* offsets, const float* smin,
+                 const float* srg, int32_t n_seg, Plan* out) {
+  std::string why;
+  if (int r = validate_batch(offsets, smin, srg, n_seg, &why)) return fail(c, r, why);
   const int64_t n = offsets[n_seg];
-  if (n > kMaxPoints) return fail(c, OCP_ERR_CAPACITY, "more than 2^30-1 points");
   const TuneTable& tt = c->tune;
   if (int r = ensure_workspace(c, n, n_seg, flag_words_for(tt, n, n_seg))) return r;
   Workspace& w = c->ws;
@@ -334,6 +355,20 @@ int batched_plan(ocp_ctx c, const int32_t* offsets, const float* smin,
   return OCP_OK;
 }
 
+// Brackets one kernel launch with events (eager builds with timing on).
+struct KScope {
+  ocp_ctx c;
+  int id;
+  bool on;
+  KScope(ocp_ctx c_, int id_, bool capturing)
+      : c(c_), id(id_), on(c_->stage_timing && !capturing) {
+    if (on) (void)hipEventRecord(c->kev[2 * id], c->stream);
+  }
+  ~KScope() {
+    if (on) (void)hipEventRecord(c->kev[2 * id + 1], c->stream);
+  }
+};
+
 OctArgs pipeline_oct_args(ocp_ctx c) {
   Workspace& w = c->ws;
   OctArgs a{};
@@ -355,15 +390,16 @@ OctArgs pipeline_oct_args(ocp_ctx c) {
   return a;
 }
 
-void enqueue_octree(ocp_ctx c, const Plan& p) {
+void enqueue_octree(ocp_ctx c, const Plan& p, bool capturing) {
   Workspace& w = c->ws;
   hipStream_t s = c->stream;
   int32_t* kalloc = w.counts + w.seg_cap;
-  launch_oct_offsets(kalloc, p.n_seg, w.oct_start, c->ctrl, w.k_cap, s);
-  launch_zero_nodes(w.nodes, c->ctrl, w.k_cap, s);
+  { KScope k(c, kKOctOffsets, capturing);
+    launch_oct_offsets(kalloc, p.n_seg, w.oct_start, c->ctrl, w.k_cap, s); }
   const OctArgs a = pipeline_oct_args(c);
-  launch_oct_scatter(a, p.sv, p.ts_node, p.tiles_node, 1 | 2, s);
-  launch_oct_finalize(a, p.sv, p.ts_node, p.tiles_node, 1 | 2 | 4 | 8, s);
+  { KScope k(c, kKOctBuild, capturing);
+    launch_oct_build(a, p.sv, p.ts_node, p.tiles_node,
+                     kOctMake | kOctLink | kOctRoot | kOctFull, s); }
 }
 
 void mark(ocp_ctx c, int k, bool capturing) {
@@ -379,28 +415,35 @@ void enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
   int32_t* n_unique = w.counts;
   int32_t* kalloc = w.counts + w.seg_cap;
   mark(c, 0, capturing);
-  launch_begin(c->ctrl, w.hist, int64_t(p.n_seg) * 1024, s);
-  launch_morton(pts, f3, w.keys_a, p.sv, p.ts_morton, p.tiles_morton, w.hist, tt, s);
+  { KScope k(c, kKBegin, capturing);
+    launch_begin(c->ctrl, w.hist, int64_t(p.n_seg) * 1024, s); }
+  { KScope k(c, kKMorton, capturing);
+    launch_morton(pts, f3, w.keys_a, p.sv, p.ts_morton, p.tiles_morton, w.hist, tt, s); }
   mark(c, 1, capturing);
   uint32_t* src = w.keys_a;
   uint32_t* dst = w.keys_b;
   for (int pass = 0; pass < 4; ++pass) {
+    KScope k(c, kKPass0 + pass, capturing);
     launch_onesweep_pass(src, dst, p.sv, p.ts_sort, p.tiles_sort, w.hist, w.flags,
                          c->ctrl, pass, tt, s);
     std::swap(src, dst);
   }
   mark(c, 2, capturing);
-  launch_fill_u32(reinterpret_cast<uint32_t*>(w.counts), 0u, int64_t(w.seg_cap) * 2, s);
-  launch_unique(w.keys_a, w.keys_b, p.sv, p.ts_scan, p.tiles_scan, w.flags, c->ctrl,
-                n_unique, kalloc, tt, s);
+  { KScope k(c, kKFill, capturing);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(w.counts), 0u, int64_t(w.seg_cap) * 2, s); }
+  { KScope k(c, kKUnique, capturing);
+    launch_unique(w.keys_a, w.keys_b, p.sv, p.ts_scan, p.tiles_scan, w.flags, c->ctrl,
+                  n_unique, kalloc, tt, s); }
   mark(c, 3, capturing);
-  launch_radix_tree(w.keys_b, w.prefix_n, w.hl, w.hr, w.lc, w.par, p.sv, p.ts_node,
-                    p.tiles_node, n_unique, 0, s);
+  { KScope k(c, kKRadixTree, capturing);
+    launch_radix_tree(w.keys_b, w.prefix_n, w.hl, w.hr, w.lc, w.par, p.sv, p.ts_node,
+                      p.tiles_node, n_unique, 0, s); }
   mark(c, 4, capturing);
-  launch_edge_scan(w.prefix_n, w.par, w.edge, w.offs, p.sv, p.ts_scan, p.tiles_scan,
-                   n_unique, 0, w.flags, c->ctrl, kalloc, 0, tt, s);
+  { KScope k(c, kKEdgeScan, capturing);
+    launch_edge_scan(w.prefix_n, w.par, w.edge, w.offs, p.sv, p.ts_scan, p.tiles_scan,
+                     n_unique, 0, w.flags, c->ctrl, kalloc, 0, tt, s); }
   mark(c, 5, capturing);
-  enqueue_octree(c, p);
+  enqueue_octree(c, p, capturing);
   mark(c, 6, capturing);
 }
 
@@ -460,11 +503,16 @@ int settle(ocp_ctx c) {
   uint32_t hc[kCtrlWords];
   OCP_HIP(c, hipMemcpyAsync(hc, c->ctrl, sizeof(hc), hipMemcpyDeviceToHost, c->stream));
   OCP_HIP(c, hipStreamSynchronize(c->stream));
+  if (hc[kCtrlTimeout] != 0) {
+    OCP_HIP(c, hipMemsetAsync(c->ctrl + kCtrlTimeout, 0, sizeof(uint32_t), c->stream));
+    OCP_HIP(c, hipStreamSynchronize(c->stream));
+    return fail(c, OCP_ERR_HIP, "a lookback wait hit its spin limit; the result is invalid");
+  }
   if (hc[kCtrlOctOk] == 0) {
     const int64_t need = hc[kCtrlOctTotal];
     clear_graphs(c);
     if (int r = alloc_nodes(c, need + need / 4 + 1024)) return r;
-    enqueue_octree(c, c->last_plan);
+    enqueue_octree(c, c->last_plan, true);
     OCP_CHECK_LAUNCH(c);
     OCP_HIP(c, hipMemcpyAsync(hc, c->ctrl, sizeof(hc), hipMemcpyDeviceToHost, c->stream));
     OCP_HIP(c, hipStreamSynchronize(c->stream));
@@ -505,6 +553,7 @@ int ocp_create(ocp_ctx* out, int device, void* stream) {
     return OCP_ERR_HIP;
   }
   for (auto& e : c->ev) (void)hipEventCreate(&e);
+  for (auto& e : c->kev) (void)hipEventCreate(&e);
   *out = c;
   return OCP_OK;
 }
@@ -518,6 +567,8 @@ int ocp_destroy(ocp_ctx c) {
   dev_free(c->s_flags);
   dev_free(c->s_hist);
   for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->kev)
     if (e) (void)hipEventDestroy(e);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -746,6 +797,7 @@ int ocp_k_MakeOctNodes(ocp_ctx c, ocp_oct_node* nodes, const int32_t* offs,
                                    nullptr, n, n_alloc);
   const SegView sv = single_view(n + 1, mn, rg);
   const int32_t tiles = static_cast<int32_t>(cdiv(n, kNodeTile));
+  // No children arrays in this call: the push-style kernels.
   launch_oct_scatter(a, sv, nullptr, tiles, 1, c->stream);
   launch_oct_finalize(a, sv, nullptr, tiles, 1 | 4, c->stream);
   OCP_CHECK_LAUNCH(c);
@@ -811,6 +863,12 @@ int ocp_build_batched(ocp_ctx c, const float* points, int float3, const int32_t*
                   graph_key("batch", points, float3, p.n_total, 0, 0, &c->last.offsets, c->tune));
 }
 
+int ocp_check_batch(const int32_t* offsets, const float* smin, const float* srg,
+                    int32_t n_seg) {
+  std::string why;
+  return validate_batch(offsets, smin, srg, n_seg, &why);
+}
+
 int ocp_set_graph_mode(ocp_ctx c, int enable) {
   if (check_ctx(c)) return OCP_ERR_INVALID;
   c->graph_mode = enable != 0;
@@ -863,6 +921,19 @@ int ocp_get_stage_ms(ocp_ctx c, float* ms7) {
   OCP_HIP(c, hipEventSynchronize(c->ev[6]));
   for (int k = 0; k < 6; ++k) OCP_HIP(c, hipEventElapsedTime(&ms7[k], c->ev[k], c->ev[k + 1]));
   OCP_HIP(c, hipEventElapsedTime(&ms7[6], c->ev[0], c->ev[6]));
+  return OCP_OK;
+}
+
+int ocp_get_kernel_ms(ocp_ctx c, float* ms, const char** names, int32_t max_kernels,
+                      int32_t* n_kernels) {
+  if (check_ctx(c) || !n_kernels) return OCP_ERR_INVALID;
+  *n_kernels = kKernelCount;
+  if (!c->ev_recorded) return fail(c, OCP_ERR_STATE, "no eager build with stage timing on");
+  OCP_HIP(c, hipEventSynchronize(c->ev[6]));
+  for (int k = 0; k < kKernelCount && k < max_kernels; ++k) {
+    if (names) names[k] = kKernelNames[k];
+    if (ms) OCP_HIP(c, hipEventElapsedTime(&ms[k], c->kev[2 * k], c->kev[2 * k + 1]));
+  }
   return OCP_OK;
 }
 
@@ -950,7 +1021,7 @@ int ocp_autotune(ocp_ctx c, const float* points, int float3, int64_t n,
     if (int r = time_cfg(t, &ms)) return r;
     if (ms < best_ms) { best_ms = ms; best = t; }
   }
-  for (int ipt : {4, 8, 16}) {
+  for (int ipt : {8, 16, 32}) {
     ocp_tune t = best;
     t.scan_ipt = ipt;
     float ms;

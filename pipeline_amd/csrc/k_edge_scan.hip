@@ -3,11 +3,11 @@
 // inclusive scan of kernels/06_partial_sum.cpp:3-15 (the reference GPU path
 // has no scan and round-trips to the host, apps/gpu_demo/main.cu:308-310).
 //
-// Tiles of 256*IPT nodes, ticket-ordered, one decoupled-lookback chain per
-// segment. Each thread handles one node per row of 256 (coalesced loads of
+// Tiles of 256*IPT nodes, one per workgroup, one decoupled-lookback chain
+// per segment. Each thread handles one node per row of 256 (coalesced loads of
 // prefix_n and parent, one gathered byte prefix_n[parent]); rows are scanned
-// with wave64 shuffles, one wave scans the 4*IPT row totals, one thread
-// looks back. Each tile also reduces max(P[i] + e[i]) into the segment's
+// with wave64 shuffles, one wave scans the 4*IPT row totals and looks back
+// 64 predecessors per round trip. Each tile also reduces max(P[i] + e[i]) into the segment's
 // octree slot count with one atomicMax.
 //
 // Modes: 0 = fused, 1 = edge count only, 2 = scan of `edge` only.
@@ -26,22 +26,15 @@ __global__ __launch_bounds__(256) void edge_scan_kernel(
   constexpr int THREADS = 256;
   constexpr int NW = THREADS / 64;
   constexpr int T = THREADS * IPT;
-  static_assert(IPT * NW <= 64, "row totals are scanned by one wave");
   __shared__ uint32_t s_rows[IPT * NW];
   __shared__ int32_t s_max[NW];
-  __shared__ int32_t s_ticket;
   __shared__ uint32_t s_excl;
 
   const int tid = threadIdx.x;
   const uint32_t w = wave_id();
   const uint32_t lane = lane_id();
   const bool scan = mode != 1;
-  int32_t t = blockIdx.x;
-  if (scan) {
-    if (tid == 0) s_ticket = static_cast<int32_t>(atomicAdd(&ctrl[kCtrlTicket0 + kStageScan], 1u));
-    __syncthreads();
-    t = s_ticket;
-  }
+  const int32_t t = blockIdx.x;  // in-order dispatch: see lookback_n
   const TileRef r = find_tile(tile_start, sv.n_seg, t);
   const int32_t n = n_unique ? n_unique[r.seg] - 1 : n_brt_host;
   const int32_t base = r.local * T;
@@ -77,20 +70,16 @@ __global__ __launch_bounds__(256) void edge_scan_kernel(
   }
   __syncthreads();
   if (w == 0) {
-    const uint32_t v = lane < IPT * NW ? s_rows[lane] : 0u;
-    const uint32_t in = wave_incl_scan(v);
-    if (lane < IPT * NW) s_rows[lane] = in - v;
-    const uint32_t total = __shfl(in, IPT * NW - 1, 64);
+    const uint32_t total = wave_scan_lds<IPT * NW>(s_rows);
+    const uint32_t tag = ctrl[kCtrlEpoch] * 8u + kStageScan;
+    if (lane == 0)
+      flag_store(&flags[t], flag_pack(tag, r.local == 0 ? kFlagInc : kFlagAgg, total));
+    uint32_t prev = 0;
+    bool late = false;
+    if (r.local > 0) prev = wave_lookback(flags, t, t - r.local, tag, &late);
     if (lane == 0) {
-      const uint32_t tag = ctrl[kCtrlEpoch] * 8u + kStageScan;
-      uint32_t prev = 0;
-      if (r.local == 0) {
-        flag_store(&flags[t], flag_pack(tag, kFlagInc, total));
-      } else {
-        flag_store(&flags[t], flag_pack(tag, kFlagAgg, total));
-        prev = lookback(flags, 1, t, tag);
-        flag_store(&flags[t], flag_pack(tag, kFlagInc, prev + total));
-      }
+      if (late) ctrl[kCtrlTimeout] = 1u;
+      if (r.local > 0) flag_store(&flags[t], flag_pack(tag, kFlagInc, prev + total));
       s_excl = prev;
     }
   }
@@ -141,7 +130,7 @@ void launch_edge_scan(const uint8_t* prefix_n, const int32_t* parent,
                       int mode, const TuneTable& tt, hipStream_t s) {
   if (total_tiles <= 0) return;
   switch (tt.scan_ipt) {
-    case 4: launch_ipt<4>(prefix_n, parent, edge, offsets, sv, tile_start, total_tiles, n_unique, n_brt_host, flags, ctrl, kalloc, mode, s); break;
+    case 32: launch_ipt<32>(prefix_n, parent, edge, offsets, sv, tile_start, total_tiles, n_unique, n_brt_host, flags, ctrl, kalloc, mode, s); break;
     case 8: launch_ipt<8>(prefix_n, parent, edge, offsets, sv, tile_start, total_tiles, n_unique, n_brt_host, flags, ctrl, kalloc, mode, s); break;
     default: launch_ipt<16>(prefix_n, parent, edge, offsets, sv, tile_start, total_tiles, n_unique, n_brt_host, flags, ctrl, kalloc, mode, s); break;
   }

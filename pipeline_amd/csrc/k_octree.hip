@@ -1,33 +1,66 @@
-// Stage 7: octree nodes from the radix tree - ProcessOctNode and
-// ProcessLinkLeaf of include/shared/oct.h:24-134, driven by
-// kernels/07_octree.cpp:5-57 - with the sequential (ascending index)
-// semantics of the reference CPU path, deterministically and without races.
+// Stage 7 of the fused build: octree nodes from the radix tree -
+// ProcessOctNode and ProcessLinkLeaf of include/shared/oct.h:24-134, driven
+// by kernels/07_octree.cpp:5-57 - with the sequential (ascending index)
+// semantics of the reference CPU path, in one pass and without atomics.
 //
-// The reference's chains overlap: offsets are an inclusive scan
+// The reference's chains overlap: the offsets P are an inclusive scan
 // (kernels/06_partial_sum.cpp) but P[i] is used as the first slot of chain
-// i, and run sequentially the last writer of every field wins. Two kernels
-// reproduce that:
+// i, so run sequentially the last writer of a field wins. The kernel gives
+// every node slot exactly one writer that computes its final 60 bytes:
 //
-// scatter (one thread per BRT node): every child-slot write of the
-//   sequential program is an atomicMax of an order key into the slot's
-//   children[c] word, used as a 32-bit unsigned scratch: make writes carry
-//   (i << 4 | j) + 1 (chain i, step j), link writes 0x80000000 | (i << 1 |
-//   side), so the maximum is the last writer and every link write beats
-//   every make write, as the link pass runs after the make pass. Child-node
-//   mask bits are atomicOr (OR commutes). The leaf mask only ever sees
-//   AND-NOT on a zeroed word (oct.h:20), so it stays 0.
-// finalize (one thread per BRT node): chain i owns slot P[i]+j when no
-//   later chain covers it (later non-empty chains start strictly later, so
-//   a short forward walk over P finds them). The owner writes corner and
-//   cell size from (i, j) and turns the children keys of its slots back
-//   into values: P[i'] + j' for a make key, left_child[i'] + side for a
-//   link key. Every slot that receives a child write is either covered by
-//   a chain or is the root slot 0, which thread 0 finalises together with
-//   the root corner (apps/cpu_demo/main.cpp:125-132).
+//  * Chain i (e[i] > 0) covers slots P[i] .. P[i]+e[i]-1. Non-empty chains
+//    start strictly later as i grows, so chain i always owns its first slot
+//    and owns slot P[i]+j unless a later chain covers it. The owner writes
+//    the corner and cell size of step j (oct.h:44-57, 76-88).
+//  * Make-phase child links into slot s come from the chains covering s:
+//    chain q writes children[c] = s-1 there (oct.h:41-51), so those links
+//    need no ordering, only the union of their child indices. The first
+//    slot P[a] of chain a additionally receives the top links of the
+//    non-empty chains whose parent walk (oct.h:61-73) ends at a, and the
+//    leaf links (oct.h:92-134) of every node whose walk ends at a. Those
+//    nodes are a's group: a and the e == 0 internal nodes below it, at most
+//    two levels deep because prefix lengths grow strictly down the tree and
+//    one octree level spans three of them. The owner visits the group
+//    downwards and keeps, per child index, the write that comes last in
+//    the sequential order: make writes by (chain, step), every link write
+//    after every make write, links by (node, side).
+//  * The root group follows parent[] rather than node 0's own children:
+//    node 0 is the reference's short-range root (04_radix_tree.cpp:63-65),
+//    and the nodes whose parent is 0 are the children of the true root.
+//  * Slots no chain covers stay zero; the chain before the gap (or the root
+//    for the slots before the first chain) writes those zeros, so the node
+//    buffer needs no zero fill.
+//  * child_leaf_mask only ever sees AND-NOT on a zeroed word (oct.h:20), so
+//    it is 0; child_node_mask is the OR of the make-phase link bits.
+//
+// Layout of the work: a workgroup of 128 threads takes 256 consecutive BRT
+// nodes and stages a packed 16-byte record {P, e, left child + leaf flags,
+// prefix length} and the code of those nodes plus 64 on either side into
+// LDS with coalesced loads. It compacts the non-empty chains of its range
+// so that every lane processes a chain. The short walks over neighbouring
+// chains and most child lookups of the group walk are LDS reads (anything
+// outside the window falls back to global memory); the backward walk stops
+// as soon as no earlier chain can reach this one, using the largest edge
+// count of the window. Nodes are assembled in registers and written with
+// three 16-byte and one 12-byte store.
+//
+// Edge counts are at most 10 (prefix lengths are at most 31 for 30-bit
+// codes), so a chain spans at most 10 slots.
 #include "ocp_internal.hpp"
 
 namespace ocp {
 namespace {
+
+constexpr int32_t kMaxEdge = 10;
+constexpr int kThreads = 128;
+constexpr int kHalo = 64;
+constexpr int kWinN = kNodeTile + 2 * kHalo;
+constexpr uint32_t kLeafL = 1u << 30;
+constexpr uint32_t kLeafR = 1u << 31;
+constexpr uint32_t kChildMask = (1u << 30) - 1u;
+
+typedef int32_t v4i __attribute__((ext_vector_type(4)));
+typedef int32_t v3i __attribute__((ext_vector_type(3)));
 
 struct SegBind {
   OctNode* nodes;
@@ -38,184 +71,450 @@ struct SegBind {
   const uint8_t* hl;
   const uint8_t* hr;
   const int32_t* lc;
-  const int32_t* par;
   const int32_t* e;
   const int32_t* p;
   float mn;
   float rg;
 };
 
-// Binds thread -> (segment, BRT node i). With root_only, thread 0 of a
-// segment that holds a single unique key (no BRT node, one root slot) is
-// bound too, with m = 0.
-__device__ __forceinline__ bool bind(const OctArgs& a, const SegView& sv,
-                                     const int32_t* tile_start, int32_t& i,
-                                     SegBind& b, bool root_only = false) {
-  if (a.ctrl && a.ctrl[kCtrlOctOk] == 0) return false;
+// One BRT node as the kernel reads it.
+struct Rec {
+  int32_t p;
+  int32_t e;
+  uint32_t lc;  // left child | kLeafL | kLeafR
+  int32_t pn;
+  __device__ __forceinline__ int32_t child() const { return static_cast<int32_t>(lc & kChildMask); }
+  __device__ __forceinline__ bool leaf(int side) const { return (lc & (side ? kLeafR : kLeafL)) != 0; }
+};
+
+struct View {
+  const SegBind& b;
+  int32_t lo;       // window [lo, lo + n) of nodes, [lo, lo + nc) of codes
+  int32_t n;
+  int32_t nc;
+  const v4i* rec;
+  const uint32_t* code;
+  __device__ __forceinline__ Rec R(int32_t q) const {
+    if (static_cast<uint32_t>(q - lo) < static_cast<uint32_t>(n)) {
+      const v4i r = rec[q - lo];
+      return {r.x, r.y, static_cast<uint32_t>(r.z), r.w};
+    }
+    const uint32_t l = static_cast<uint32_t>(b.lc[q]) | (b.hl[q] ? kLeafL : 0u) |
+                       (b.hr[q] ? kLeafR : 0u);
+    return {b.p[q], b.e[q], l, static_cast<int32_t>(b.pn[q])};
+  }
+  __device__ __forceinline__ int32_t P(int32_t q) const {
+    return static_cast<uint32_t>(q - lo) < static_cast<uint32_t>(n) ? rec[q - lo].x : b.p[q];
+  }
+  __device__ __forceinline__ int32_t E(int32_t q) const {
+    return static_cast<uint32_t>(q - lo) < static_cast<uint32_t>(n) ? rec[q - lo].y : b.e[q];
+  }
+  __device__ __forceinline__ uint32_t CODE(int32_t q) const {
+    return static_cast<uint32_t>(q - lo) < static_cast<uint32_t>(nc) ? code[q - lo] : b.codes[q];
+  }
+};
+
+__device__ __forceinline__ uint32_t child_of(uint32_t code, int32_t level) {
+  return (code >> (static_cast<uint32_t>(kMortonBits - 3 * level) & 31u)) & 7u;
+}
+__device__ __forceinline__ uint32_t delta(uint32_t a, uint32_t b) {
+  const uint32_t x = a ^ b;
+  return ((x ? static_cast<uint32_t>(__clz(static_cast<int>(x))) : 32u) - 1u) & 0xffu;
+}
+
+// Per-child-index last writer of one slot, kept in registers.
+struct SlotChildren {
+  uint32_t key[8];
+  int32_t val[8];
+  uint32_t node_mask;
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      key[c] = 0;
+      val[c] = 0;
+    }
+    node_mask = 0;
+  }
+  __device__ __forceinline__ void put(uint32_t which, uint32_t k, int32_t v) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      if (static_cast<uint32_t>(c) == which && k > key[c]) {
+        key[c] = k;
+        val[c] = v;
+      }
+    }
+  }
+};
+
+__device__ __forceinline__ uint32_t make_key(int32_t chain, int32_t step) {
+  return static_cast<uint32_t>((chain << 4) | step) + 1u;
+}
+__device__ __forceinline__ uint32_t link_key(int32_t node, int side) {
+  return 0x80000000u | static_cast<uint32_t>((node << 1) | side);
+}
+
+// Child `side` of group node x (record rx): a leaf link, a top link of a
+// non-empty chain, or (returned) an e == 0 internal node joining the group.
+__device__ __forceinline__ int32_t visit_child(const View& v, int32_t x, const Rec& rx,
+                                               int side, SlotChildren& sc, Rec* out) {
+  const int32_t ch = rx.child() + side;
+  if (rx.leaf(side)) {
+    if (ch < 0 || ch > v.b.m) return -1;  // leaves index the m+1 codes
+    sc.put(child_of(v.CODE(ch), rx.pn / 3 + 1), link_key(x, side), ch);
+    return -1;
+  }
+  if (ch <= 0 || ch >= v.b.m) return -1;
+  const Rec ry = v.R(ch);
+  if (ry.e == 0) {
+    *out = ry;
+    return ch;
+  }
+  if (ry.e > 0) {
+    const uint32_t w = child_of(v.CODE(ch), ry.pn / 3 - ry.e + 1);
+    sc.put(w, make_key(ch, ry.e - 1), ry.p + ry.e - 1);
+    sc.node_mask |= 1u << w;
+  }
+  return -1;
+}
+
+// Members below the given ones, down to two further levels.
+__device__ __forceinline__ void visit_members(const View& v, int32_t m0, const Rec& r0,
+                                              int32_t m1, const Rec& r1, SlotChildren& sc) {
+  int32_t l2[4] = {-1, -1, -1, -1};
+  Rec q2[4];
+  if (m0 >= 0) {
+    l2[0] = visit_child(v, m0, r0, 0, sc, &q2[0]);
+    l2[1] = visit_child(v, m0, r0, 1, sc, &q2[1]);
+  }
+  if (m1 >= 0) {
+    l2[2] = visit_child(v, m1, r1, 0, sc, &q2[2]);
+    l2[3] = visit_child(v, m1, r1, 1, sc, &q2[3]);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (l2[q] < 0) continue;
+    Rec g[2];
+    const int32_t g0 = visit_child(v, l2[q], q2[q], 0, sc, &g[0]);
+    const int32_t g1 = visit_child(v, l2[q], q2[q], 1, sc, &g[1]);
+    // Below that every child is a leaf or starts a chain: prefix lengths
+    // would otherwise exceed the three values of one octree level.
+    if (g0 >= 0) {
+      Rec dummy;
+      (void)visit_child(v, g0, g[0], 0, sc, &dummy);
+      (void)visit_child(v, g0, g[0], 1, sc, &dummy);
+    }
+    if (g1 >= 0) {
+      Rec dummy;
+      (void)visit_child(v, g1, g[1], 0, sc, &dummy);
+      (void)visit_child(v, g1, g[1], 1, sc, &dummy);
+    }
+  }
+}
+
+// A whole node in registers: children, corner, cell size, masks.
+struct NodeOut {
+  int32_t w[15];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int c = 0; c < 15; ++c) w[c] = 0;
+  }
+  __device__ __forceinline__ void geometry(uint32_t code, int32_t level, int32_t root_level,
+                                           float mn, float rg) {
+    const uint32_t shift = static_cast<uint32_t>(kMortonBits - 3 * level) & 31u;
+    const uint32_t c = (code >> shift) << shift;
+    w[8] = __float_as_int(decode_axis(c, 0, mn, rg));
+    w[9] = __float_as_int(decode_axis(c, 1, mn, rg));
+    w[10] = __float_as_int(decode_axis(c, 2, mn, rg));
+    w[11] = __float_as_int(1.0f);
+    const uint32_t sh = static_cast<uint32_t>(level - root_level) & 31u;
+    w[12] = __float_as_int(rg / static_cast<float>(static_cast<int32_t>(1u << sh)));
+  }
+  __device__ __forceinline__ void children(const SlotChildren& sc) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) w[c] = sc.val[c];
+    w[13] = static_cast<int32_t>(sc.node_mask);
+    w[14] = 0;
+  }
+  __device__ __forceinline__ void store(OctNode* nd) const {
+    int32_t* p = reinterpret_cast<int32_t*>(nd);
+    const v4i a = {w[0], w[1], w[2], w[3]};
+    const v4i b = {w[4], w[5], w[6], w[7]};
+    const v4i c = {w[8], w[9], w[10], w[11]};
+    const v3i d = {w[12], w[13], w[14]};
+    __builtin_memcpy(p, &a, 16);
+    __builtin_memcpy(p + 4, &b, 16);
+    __builtin_memcpy(p + 8, &c, 16);
+    __builtin_memcpy(p + 12, &d, 12);
+  }
+};
+
+__device__ __forceinline__ void store_zeros(OctNode* nodes, int32_t from, int32_t to) {
+  NodeOut z;
+  z.zero();
+  for (int32_t s = from; s < to; ++s) z.store(nodes + s);
+}
+
+__device__ __forceinline__ void build_root(const View& v, int32_t root_level) {
+  const SegBind& b = v.b;
+  SlotChildren sc;
+  sc.init();
+  if (b.m >= 1) {
+    // Node 0's own leaves link into the root slot.
+    const Rec r0 = v.R(0);
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      if (r0.leaf(side)) {
+        const int32_t leaf = r0.child() + side;
+        if (leaf >= 0 && leaf <= b.m)
+          sc.put(child_of(v.CODE(leaf), r0.pn / 3 + 1), link_key(0, side), leaf);
+      }
+    }
+    // The group grows through the true root's internal children, the nodes
+    // whose parent entry is 0: the split of [0, n] (04_radix_tree.cpp:94-104
+    // with d = +1, l = n).
+    const int32_t n = b.m;
+    const uint32_t c0 = v.CODE(0);
+    const uint32_t dn = delta(c0, v.CODE(n));
+    const int32_t lo = 31 - static_cast<int32_t>(__clz(n));
+    const int32_t lg = lo + ((1u << lo) < static_cast<uint32_t>(n));
+    const int64_t max_div = int64_t(1) << lg;
+    int32_t s = 0;
+    int64_t div = 2;
+    for (uint32_t t = (static_cast<uint32_t>(n) + 1u) / 2u; div <= max_div;
+         div <<= 1, t = static_cast<uint32_t>((n + div - 1) / div)) {
+      const int32_t st = s + static_cast<int32_t>(t);
+      if (st <= n && delta(c0, v.CODE(st)) > dn) s = st;
+    }
+    int32_t mem[2] = {-1, -1};
+    Rec mr[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int32_t y = s + q;
+      const bool internal = q == 0 ? (s != 0) : (s + 1 != n);
+      if (!internal || y <= 0 || y >= n) continue;
+      const Rec ry = v.R(y);
+      if (ry.e == 0) {
+        mem[q] = y;
+        mr[q] = ry;
+      } else if (ry.e > 0) {
+        const uint32_t w = child_of(v.CODE(y), ry.pn / 3 - ry.e + 1);
+        sc.put(w, make_key(y, ry.e - 1), ry.p + ry.e - 1);
+        sc.node_mask |= 1u << w;
+      }
+    }
+    // Members at depth 1 and below, like any group.
+    int32_t l2[4] = {-1, -1, -1, -1};
+    Rec q2[4];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      if (mem[q] < 0) continue;
+      l2[2 * q] = visit_child(v, mem[q], mr[q], 0, sc, &q2[2 * q]);
+      l2[2 * q + 1] = visit_child(v, mem[q], mr[q], 1, sc, &q2[2 * q + 1]);
+    }
+    visit_members(v, l2[0], q2[0], l2[1], q2[1], sc);
+    visit_members(v, l2[2], q2[2], l2[3], q2[3], sc);
+  }
+  NodeOut nd;
+  nd.children(sc);
+  const uint32_t shift = static_cast<uint32_t>(kMortonBits - 3 * root_level) & 31u;
+  const uint32_t c = (v.CODE(0) >> shift) << shift;
+  nd.w[8] = __float_as_int(decode_axis(c, 0, b.mn, b.rg));
+  nd.w[9] = __float_as_int(decode_axis(c, 1, b.mn, b.rg));
+  nd.w[10] = __float_as_int(decode_axis(c, 2, b.mn, b.rg));
+  nd.w[11] = __float_as_int(1.0f);
+  nd.w[12] = __float_as_int(b.rg);
+  nd.store(b.nodes);
+  // Slots before the first chain are never written by the reference.
+  int32_t first = b.k;
+  for (int32_t q = 1; q < b.m; ++q) {
+    if (v.E(q) > 0) {
+      first = v.P(q);
+      break;
+    }
+  }
+  store_zeros(b.nodes, 1, min(first, b.k));
+}
+
+// Child-index bits of step links into slot j (1..9) of a chain.
+struct StepBits {
+  uint64_t lo = 0;  // slots 0..7, 8 bits each
+  uint32_t hi = 0;  // slots 8..9
+  __device__ __forceinline__ void set(int32_t j, uint32_t w) {
+    if (j < 8) lo |= static_cast<uint64_t>(1u << w) << (8 * j);
+    else hi |= (1u << w) << (8 * (j - 8));
+  }
+  __device__ __forceinline__ uint32_t get(int32_t j) const {
+    return j < 8 ? static_cast<uint32_t>(lo >> (8 * j)) & 0xffu
+                 : (hi >> (8 * (j - 8))) & 0xffu;
+  }
+};
+
+__device__ __forceinline__ void build_chain(const View& v, int32_t i, int32_t root_level,
+                                            int32_t emax_win, int32_t reach_before) {
+  const SegBind& b = v.b;
+  const Rec ri = v.R(i);
+  const int32_t nn = min(ri.e, kMaxEdge);
+  const int32_t p_i = ri.p;
+  const int32_t last = p_i + nn - 1;
+
+  // Later chains: which slots stay ours, and where the next chain starts.
+  uint32_t owned = (1u << nn) - 1u;
+  int32_t next_start = b.k;
+  bool found_next = false;
+  for (int32_t q = i + 1; q < b.m; ++q) {
+    const int32_t pq = v.P(q);
+    if (found_next && pq > last) break;
+    const int32_t eq = v.E(q);
+    if (eq <= 0) continue;
+    if (!found_next) {
+      next_start = pq;
+      found_next = true;
+    }
+    if (pq > last) break;
+    const int32_t hi = min(pq + eq - 1, last);
+    for (int32_t s = pq; s <= hi; ++s) owned &= ~(1u << (s - p_i));
+  }
+
+  // Step links into this chain's slots: its own, and those of earlier chains
+  // that still cover them; their furthest end bounds the following gap.
+  // Chains before the window end by reach_before; chains inside it by their
+  // start plus the window's largest edge count.
+  const uint32_t code_i = v.CODE(i);
+  const int32_t lvl_i = ri.pn / 3;
+  StepBits bits;
+  SlotChildren sc;
+  sc.init();
+  for (int32_t t = 0; t + 1 < nn; ++t) bits.set(t + 1, child_of(code_i, lvl_i - t));
+  int32_t reach = p_i + nn;
+  const bool before_matters = reach_before > p_i;
+  for (int32_t q = i - 1; q >= 1; --q) {
+    const int32_t pq = v.P(q);
+    const bool inside = q >= v.lo;
+    if (inside ? (pq + emax_win <= p_i && !before_matters) : (pq + kMaxEdge <= p_i)) break;
+    const int32_t eq = min(v.E(q), kMaxEdge);
+    if (eq <= 0) continue;
+    reach = max(reach, pq + eq);
+    if (pq + eq - 1 < p_i) continue;
+    const uint32_t code_q = v.CODE(q);
+    const int32_t lvl_q = v.R(q).pn / 3;
+    const int32_t s_hi = min(pq + eq - 1, last);
+    for (int32_t s = max(p_i, pq + 1); s <= s_hi; ++s) {
+      const int32_t t = s - pq - 1;  // chain q's step t writes slot pq+t+1
+      const uint32_t w = child_of(code_q, lvl_q - t);
+      if (s == p_i) {
+        sc.put(w, make_key(q, t), s - 1);
+        sc.node_mask |= 1u << w;
+      } else {
+        bits.set(s - p_i, w);
+      }
+    }
+  }
+
+  // First slot: the earlier chains' step links, then the group's top links
+  // and leaf links.
+  Rec r0, r1;
+  const int32_t m0 = visit_child(v, i, ri, 0, sc, &r0);
+  const int32_t m1 = visit_child(v, i, ri, 1, sc, &r1);
+  if (m0 >= 0 || m1 >= 0) visit_members(v, m0, r0, m1, r1, sc);
+  if (p_i > 0 && p_i < b.k) {
+    NodeOut nd;
+    nd.children(sc);
+    nd.geometry(code_i, lvl_i, root_level, b.mn, b.rg);
+    nd.store(b.nodes + p_i);
+  }
+  for (int32_t j = 1; j < nn; ++j) {
+    if (!((owned >> j) & 1u)) continue;
+    const int32_t s = p_i + j;
+    if (s <= 0 || s >= b.k) continue;
+    NodeOut nd;
+    const uint32_t w = bits.get(j);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) nd.w[c] = ((w >> c) & 1u) ? s - 1 : 0;
+    nd.w[13] = static_cast<int32_t>(w);
+    nd.w[14] = 0;
+    nd.geometry(code_i, lvl_i - j, root_level, b.mn, b.rg);
+    nd.store(b.nodes + s);
+  }
+  store_zeros(b.nodes, max(reach, 1), min(next_start, b.k));
+}
+
+__global__ __launch_bounds__(kThreads) void oct_build_kernel(
+    OctArgs a, SegView sv, const int32_t* __restrict__ tile_start) {
+  __shared__ v4i s_rec[kWinN];
+  __shared__ uint32_t s_code[kWinN];
+  __shared__ int32_t s_list[kNodeTile];
+  __shared__ int32_t s_count;
+  __shared__ int32_t s_emax;
+
+  if (a.ctrl && a.ctrl[kCtrlOctOk] == 0) return;
   const TileRef r = find_tile(tile_start, sv.n_seg, blockIdx.x);
+  SegBind b;
   b.m = a.n_unique ? a.n_unique[r.seg] - 1 : a.n_brt_host;
-  i = r.local * kNodeTile + static_cast<int32_t>(threadIdx.x);
-  if (i >= b.m && !(root_only && i == 0 && b.m == 0)) return false;
+  const int32_t i0 = r.local * kNodeTile;
+  // Nothing to do beyond the nodes; a single-key segment still has its root.
+  if (i0 >= b.m && !(i0 == 0 && b.m == 0)) return;
   const int32_t sb = seg_begin(sv, r.seg);
   b.nodes = a.nodes + (a.oct_start ? a.oct_start[r.seg] : 0);
   b.k = a.kalloc ? a.kalloc[r.seg] : a.k_host;
+  if (b.k < 1) return;
   b.codes = a.codes + sb;
   b.pn = a.prefix_n + sb;
   b.hl = a.has_leaf_left + sb;
   b.hr = a.has_leaf_right + sb;
   b.lc = a.left_child + sb;
-  b.par = a.parent + sb;
   b.e = a.edge + sb;
   b.p = a.offsets + sb;
   b.mn = seg_min(sv, r.seg);
   b.rg = seg_range(sv, r.seg);
-  return true;
-}
 
-__device__ __forceinline__ uint32_t child_of(uint32_t code, int32_t level) {
-  return (code >> (static_cast<uint32_t>(kMortonBits - 3 * level) & 31u)) & 7u;
-}
-
-__device__ __forceinline__ void put_key(const SegBind& b, int32_t slot,
-                                        uint32_t which, uint32_t key,
-                                        bool node_bit) {
-  if (slot < 0 || slot >= b.k) return;
-  OctNode* nd = b.nodes + slot;
-  atomicMax(reinterpret_cast<uint32_t*>(&nd->children[which]), key);
-  if (node_bit) atomicOr(&nd->child_node_mask, 1 << which);
-}
-
-__global__ __launch_bounds__(kNodeTile) void oct_scatter_kernel(
-    OctArgs a, SegView sv, const int32_t* __restrict__ tile_start, int phases) {
-  int32_t i;
-  SegBind b;
-  if (!bind(a, sv, tile_start, i, b)) return;
-  if ((phases & 1) && i >= 1) {
-    const int32_t n_new = b.e[i];
-    if (n_new > 0) {
-      const int32_t p_i = b.p[i];
-      const int32_t lvl = b.pn[i] / 3;
-      const uint32_t code = b.codes[i];
-      for (int32_t j = 0; j < n_new - 1; ++j)
-        put_key(b, p_i + j + 1, child_of(code, lvl - j),
-                static_cast<uint32_t>((i << 4) | j) + 1u, true);
-      // oct.h:61-73: first ancestor with edges, at most 31 hops.
-      int32_t rt = b.par[i];
-      int32_t counter = 0;
-      while (b.e[rt] == 0) {
-        rt = b.par[rt];
-        if (++counter > 30) break;
-      }
-      put_key(b, b.p[rt], child_of(code, lvl - n_new + 1),
-              static_cast<uint32_t>((i << 4) | (n_new - 1)) + 1u, true);
+  const int32_t lo = max(0, i0 - kHalo);
+  const int32_t wn = max(0, min(b.m, i0 + kNodeTile + kHalo) - lo);
+  const int32_t wnc = max(0, min(b.m + 1, i0 + kNodeTile + kHalo) - lo);
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    s_count = 0;
+    s_emax = 0;
+  }
+  __syncthreads();
+  int32_t emax = 0;
+  for (int q = tid; q < wnc; q += kThreads) {
+    const int32_t g = lo + q;
+    s_code[q] = b.codes[g];
+    if (q < wn) {
+      const int32_t e = b.e[g];
+      const uint32_t l = static_cast<uint32_t>(b.lc[g]) | (b.hl[g] ? kLeafL : 0u) |
+                         (b.hr[g] ? kLeafR : 0u);
+      s_rec[q] = v4i{b.p[g], e, static_cast<int32_t>(l), static_cast<int32_t>(b.pn[g])};
+      emax = max(emax, e);
     }
   }
-  if (phases & 2) {
-    const int32_t leaf_level = b.pn[i] / 3 + 1;
-    int32_t anc = -1;
+  emax = wave_incl_max(emax);
+  if (lane_id() == 0) atomicMax(&s_emax, emax);
+  __syncthreads();
+  const View v{b, lo, wn, wnc, s_rec, s_code};
+  // Compact the block's work: the root and the non-empty chains.
 #pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      const bool has = side == 0 ? b.hl[i] : b.hr[i];
-      if (!has) continue;
-      const int32_t leaf = b.lc[i] + side;
-      const uint32_t which = child_of(b.codes[leaf], leaf_level);
-      if (anc < 0) {
-        // oct.h:111-113 stopped at the root and after 64 hops.
-        anc = i;
-        int32_t hops = 0;
-        while (b.e[anc] == 0 && anc != 0 && hops < 64) {
-          anc = b.par[anc];
-          ++hops;
-        }
-      }
-      const int32_t slot = b.p[anc];
-      put_key(b, slot, which, 0x80000000u | static_cast<uint32_t>((i << 1) | side), false);
-      if ((phases & 4) && slot >= 0 && slot < b.k)
-        atomicAnd(&b.nodes[slot].child_leaf_mask, ~(1 << which));
-    }
+  for (int h = 0; h < kNodeTile / kThreads; ++h) {
+    const int32_t i = i0 + h * kThreads + tid;
+    const bool work = (i == 0) || (i < b.m && v.E(i) > 0);
+    const uint64_t bal = __ballot(work);
+    uint32_t base = 0;
+    if (lane_id() == 0 && bal)
+      base = atomicAdd(reinterpret_cast<uint32_t*>(&s_count), static_cast<uint32_t>(__popcll(bal)));
+    base = __shfl(base, 0, 64);
+    if (work) s_list[base + __popcll(bal & lanemask_lt())] = i;
   }
-}
-
-__device__ __forceinline__ void resolve_children(const SegBind& b, int32_t slot,
-                                                 int resolve) {
-  int32_t* ch = b.nodes[slot].children;
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const uint32_t v = static_cast<uint32_t>(ch[c]);
-    if (v == 0) continue;
-    if (v & 0x80000000u) {
-      if (resolve & 2) {
-        const uint32_t q = v & 0x7fffffffu;
-        ch[c] = b.lc[q >> 1] + static_cast<int32_t>(q & 1u);
-      }
-    } else if (resolve & 1) {
-      const uint32_t q = v - 1u;
-      ch[c] = b.p[q >> 4] + static_cast<int32_t>(q & 15u);
-    }
-  }
-}
-
-__device__ __forceinline__ void write_geometry(const SegBind& b, int32_t slot,
-                                               uint32_t code, int32_t level,
-                                               int32_t root_level) {
-  const uint32_t shift = static_cast<uint32_t>(kMortonBits - 3 * level) & 31u;
-  const uint32_t c = (code >> shift) << shift;
-  OctNode* nd = b.nodes + slot;
-  nd->corner[0] = decode_axis(c, 0, b.mn, b.rg);
-  nd->corner[1] = decode_axis(c, 1, b.mn, b.rg);
-  nd->corner[2] = decode_axis(c, 2, b.mn, b.rg);
-  nd->corner[3] = 1.0f;
-  const uint32_t sh = static_cast<uint32_t>(level - root_level) & 31u;
-  nd->cell_size = b.rg / static_cast<float>(static_cast<int32_t>(1u << sh));
-}
-
-__global__ __launch_bounds__(kNodeTile) void oct_finalize_kernel(
-    OctArgs a, SegView sv, const int32_t* __restrict__ tile_start,
-    int resolve) {
-  int32_t i;
-  SegBind b;
-  if (!bind(a, sv, tile_start, i, b, true)) return;
-  const int32_t root_level = b.m >= 1 ? b.pn[0] / 3 : 0;
-  if (i == 0) {
-    if (b.k < 1) return;
-    if (resolve & 8) {
-      // Root: corner of codes[0]'s root-level prefix, cell size = range.
-      const uint32_t shift = static_cast<uint32_t>(kMortonBits - 3 * root_level) & 31u;
-      const uint32_t c = (b.codes[0] >> shift) << shift;
-      OctNode* nd = b.nodes;
-      nd->corner[0] = decode_axis(c, 0, b.mn, b.rg);
-      nd->corner[1] = decode_axis(c, 1, b.mn, b.rg);
-      nd->corner[2] = decode_axis(c, 2, b.mn, b.rg);
-      nd->corner[3] = 1.0f;
-      nd->cell_size = b.rg;
-    }
-    resolve_children(b, 0, resolve);
-    return;
-  }
-  const int32_t n_new = b.e[i];
-  if (n_new <= 0) return;
-  const int32_t p_i = b.p[i];
-  const int32_t last = p_i + n_new - 1;
-  uint32_t owned = (n_new >= 32) ? ~0u : ((1u << n_new) - 1u);
-  for (int32_t k = i + 1; k < b.m && owned; ++k) {
-    const int32_t pk = b.p[k];
-    if (pk > last) break;
-    const int32_t ek = b.e[k];
-    if (ek <= 0) continue;
-    const int32_t lo = max(pk, p_i) - p_i;
-    const int32_t hi = min(pk + ek - 1, last) - p_i;
-    for (int32_t q = lo; q <= hi; ++q) owned &= ~(1u << q);
-  }
-  const int32_t lvl = b.pn[i] / 3;
-  const uint32_t code = b.codes[i];
-  for (int32_t j = 0; j < n_new && j < 32; ++j) {
-    if (!((owned >> j) & 1u)) continue;
-    const int32_t slot = p_i + j;
-    // Slot 0 belongs to the root thread (chains start at P[i] >= e[i] > 0
-    // for any tree the earlier stages produce).
-    if (slot <= 0 || slot >= b.k) continue;
-    if (resolve & 4) write_geometry(b, slot, code, lvl - j, root_level);
-    resolve_children(b, slot, resolve);
+  __syncthreads();
+  const int32_t root_level = b.m >= 1 ? v.R(0).pn / 3 : 0;
+  const int32_t emax_win = min(s_emax, kMaxEdge);
+  // Chains before the window start no later than P[lo] and end within
+  // kMaxEdge slots of their start.
+  const int32_t reach_before = (lo > 0 && wn > 0) ? v.P(lo) + kMaxEdge : 0;
+  const int32_t cnt = s_count;
+  for (int q = tid; q < cnt; q += kThreads) {
+    const int32_t node = s_list[q];
+    if (node == 0) build_root(v, root_level);
+    else build_chain(v, node, root_level, emax_win, reach_before);
   }
 }
 
@@ -243,18 +542,16 @@ void launch_oct_root(OctNode* nodes, const uint32_t* codes,
   oct_root_kernel<<<1, 64, 0, s>>>(nodes, codes, prefix_n, n_brt, mn, rg);
 }
 
-void launch_oct_scatter(const OctArgs& a, const SegView& sv,
-                        const int32_t* tile_start, int32_t total_tiles,
-                        int phases, hipStream_t s) {
+void launch_oct_build(const OctArgs& a, const SegView& sv,
+                      const int32_t* tile_start, int32_t total_tiles, int mode,
+                      hipStream_t s) {
+  (void)mode;  // the fused build always writes every slot
   if (total_tiles <= 0) return;
-  oct_scatter_kernel<<<total_tiles, kNodeTile, 0, s>>>(a, sv, tile_start, phases);
-}
-
-void launch_oct_finalize(const OctArgs& a, const SegView& sv,
-                         const int32_t* tile_start, int32_t total_tiles,
-                         int resolve, hipStream_t s) {
-  if (total_tiles <= 0) return;
-  oct_finalize_kernel<<<total_tiles, kNodeTile, 0, s>>>(a, sv, tile_start, resolve);
+  // The group walk reads the children arrays.
+  if (!a.has_leaf_left || !a.has_leaf_right || !a.left_child || !a.codes ||
+      !a.prefix_n || !a.edge || !a.offsets || !a.nodes)
+    return;
+  oct_build_kernel<<<total_tiles, kThreads, 0, s>>>(a, sv, tile_start);
 }
 
 }  // namespace ocp

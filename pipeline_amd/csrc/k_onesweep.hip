@@ -5,8 +5,10 @@
 // gives the same bytes).
 //
 // One pass = one launch of onesweep_kernel over tiles of WAVES*64*KPT keys.
-//   1. A ticket counter hands out tiles in order (forward progress for the
-//      lookback: every earlier tile is already resident).
+//   1. Tile = workgroup id. Workgroups are dispatched in id order on each
+//      XCD, so the lowest unfinished tile is always resident and the
+//      lookback makes progress without a shared ticket counter (a single
+//      device-scope counter serialises at one atomic per tile).
 //   2. Keys are loaded wave-contiguously (lane-strided, coalesced) and
 //      ranked per wave with a 64-lane multisplit: eight ballots build the
 //      mask of lanes holding the same digit; popcount below the lane gives a
@@ -38,15 +40,13 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
   __shared__ uint32_t s_dstart[256];
   __shared__ int32_t s_dst[256];
   __shared__ uint32_t s_tmp[WAVES];
-  __shared__ int32_t s_ticket;
 
   const int tid = threadIdx.x;
   const uint32_t w = wave_id();
   const uint32_t lane = lane_id();
-  if (tid == 0) s_ticket = static_cast<int32_t>(atomicAdd(&ctrl[kCtrlTicket0 + pass], 1u));
   for (int i = tid; i < WAVES * 256; i += THREADS) s_whist[i] = 0;
   __syncthreads();
-  const int32_t t = s_ticket;
+  const int32_t t = blockIdx.x;
   const uint32_t tag = ctrl[kCtrlEpoch] * 8u + static_cast<uint32_t>(pass);
   const TileRef r = find_tile(tile_start, sv.n_seg, t);
   const int32_t sb = seg_begin(sv, r.seg);
@@ -108,7 +108,9 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
     s_dstart[tid] = dstart;
     uint32_t prev = 0;
     if (r.local > 0) {
-      prev = lookback(flags + tid, 256, t, tag);
+      bool late = false;
+      prev = lookback_n<16>(flags + tid, 256, t, t - r.local, tag, &late);
+      if (late) ctrl[kCtrlTimeout] = 1u;
       flag_store(&flags[static_cast<int64_t>(t) * 256 + tid],
                  flag_pack(tag, kFlagInc, prev + tile_cnt));
     }

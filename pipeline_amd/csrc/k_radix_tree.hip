@@ -38,8 +38,22 @@ __device__ __forceinline__ uint32_t ceil_div(uint32_t a, uint32_t b) {
   return (a + b - 1) / b;
 }
 
+// Codes of a segment, served from an LDS window [lo, lo + n) when the
+// index falls inside it. Most searches of a node stay close to the node.
+struct Codes {
+  const uint32_t* g;
+  const uint32_t* w;
+  int32_t lo;
+  int32_t n;
+  __device__ __forceinline__ uint32_t operator[](int32_t q) const {
+    return static_cast<uint32_t>(q - lo) < static_cast<uint32_t>(n) ? w[q - lo] : g[q];
+  }
+};
+
+constexpr int kHalo = 256;
+
 // Split search of 04_radix_tree.cpp:94-104 for node i with far end j.
-__device__ __forceinline__ int32_t split_search(const uint32_t* codes,
+__device__ __forceinline__ int32_t split_search(const Codes& codes,
                                                 int32_t i, int32_t d,
                                                 int32_t l, int32_t n,
                                                 uint32_t delta_node) {
@@ -62,13 +76,22 @@ __global__ __launch_bounds__(256) void radix_tree_kernel(
     int32_t* __restrict__ lc_all, int32_t* __restrict__ par_all, SegView sv,
     const int32_t* __restrict__ tile_start, const int32_t* __restrict__ n_unique,
     int32_t n_brt_host) {
+  __shared__ uint32_t s_codes[256 + 2 * kHalo];
   const TileRef r = find_tile(tile_start, sv.n_seg, blockIdx.x);
   const int32_t n = n_unique ? n_unique[r.seg] - 1 : n_brt_host;
-  const int32_t i = r.local * 256 + static_cast<int32_t>(threadIdx.x);
-  if (i >= n) return;
+  const int32_t i0 = r.local * 256;
+  if (i0 >= n) return;
   const int32_t sb = seg_begin(sv, r.seg);
-  const uint32_t* codes = codes_all + sb;
   int32_t* parent = par_all + sb;
+  // Stage codes [i0 - halo, i0 + 256 + halo) of the n+1 codes.
+  const int32_t lo = max(0, i0 - kHalo);
+  const int32_t hi = min(n + 1, i0 + 256 + kHalo);
+  for (int32_t q = lo + static_cast<int32_t>(threadIdx.x); q < hi; q += 256)
+    s_codes[q - lo] = codes_all[sb + q];
+  __syncthreads();
+  const Codes codes{codes_all + sb, s_codes, lo, hi - lo};
+  const int32_t i = i0 + static_cast<int32_t>(threadIdx.x);
+  if (i >= n) return;
 
   const uint32_t code_i = codes[i];
   int32_t d;

@@ -2,11 +2,11 @@
 // keep keys[i] when it differs from keys[i-1]; the reference GPU version,
 // kernels/cuda/03_unique.cu:5-24, runs on one thread).
 //
-// Tiles of 256*IPT keys, ticket-ordered. A key is kept when it is the first
+// Tiles of 256*IPT keys, one per workgroup. A key is kept when it is the first
 // of its segment or differs from its predecessor. Per row of 64 keys a
 // ballot gives the keep mask, the lane's rank (mbcnt) and the row count; one
-// wave scans the 4*IPT row counts; one thread does the decoupled lookback of
-// the tile total. Kept keys are compacted in LDS, then written with
+// wave scans the 4*IPT row counts and does the decoupled lookback of the
+// tile total, 64 predecessors per round trip. Kept keys are compacted in LDS, then written with
 // consecutive threads on consecutive output slots. The segment's last tile
 // writes U and seeds the octree slot count (1 when U >= 1).
 //
@@ -29,19 +29,16 @@ __global__ __launch_bounds__(256) void unique_kernel(
   constexpr int THREADS = 256;
   constexpr int NW = THREADS / 64;
   constexpr int T = THREADS * IPT;
-  static_assert(IPT * NW <= 64, "row counts are scanned by one wave");
   __shared__ uint32_t s_keys[T];
   __shared__ uint32_t s_rows[IPT * NW];
-  __shared__ int32_t s_ticket;
+  __shared__ uint64_t s_mask[IPT * NW];
   __shared__ uint32_t s_excl;
   __shared__ uint32_t s_total;
 
   const int tid = threadIdx.x;
   const uint32_t w = wave_id();
   const uint32_t lane = lane_id();
-  if (tid == 0) s_ticket = static_cast<int32_t>(atomicAdd(&ctrl[kCtrlTicket0 + kStageUnique], 1u));
-  __syncthreads();
-  const int32_t t = s_ticket;
+  const int32_t t = blockIdx.x;  // in-order dispatch: see lookback_n
   const uint32_t tag = ctrl[kCtrlEpoch] * 8u + kStageUnique;
   const TileRef r = find_tile(tile_start, sv.n_seg, t);
   const int32_t sb = seg_begin(sv, r.seg);
@@ -59,7 +56,6 @@ __global__ __launch_bounds__(256) void unique_kernel(
   const uint32_t before = (r.local > 0) ? in[static_cast<int64_t>(base) - 1] : 0u;
   __syncthreads();
 
-  uint64_t keep_mask[IPT];
 #pragma unroll
   for (int k = 0; k < IPT; ++k) {
     const int32_t idx = k * THREADS + tid;
@@ -70,24 +66,23 @@ __global__ __launch_bounds__(256) void unique_kernel(
       else
         keep = keys[k] != s_keys[idx - 1];
     }
-    keep_mask[k] = __ballot(keep);
-    if (lane == 0) s_rows[k * NW + w] = static_cast<uint32_t>(__popcll(keep_mask[k]));
+    const uint64_t m = __ballot(keep);
+    if (lane == 0) {
+      s_rows[k * NW + w] = static_cast<uint32_t>(__popcll(m));
+      s_mask[k * NW + w] = m;
+    }
   }
   __syncthreads();
   if (w == 0) {
-    const uint32_t v = lane < IPT * NW ? s_rows[lane] : 0u;
-    const uint32_t incl = wave_incl_scan(v);
-    if (lane < IPT * NW) s_rows[lane] = incl - v;
-    const uint32_t total = __shfl(incl, IPT * NW - 1, 64);
+    const uint32_t total = wave_scan_lds<IPT * NW>(s_rows);
+    if (lane == 0)
+      flag_store(&flags[t], flag_pack(tag, r.local == 0 ? kFlagInc : kFlagAgg, total));
+    uint32_t prev = 0;
+    bool late = false;
+    if (r.local > 0) prev = wave_lookback(flags, t, t - r.local, tag, &late);
     if (lane == 0) {
-      uint32_t prev = 0;
-      if (r.local == 0) {
-        flag_store(&flags[t], flag_pack(tag, kFlagInc, total));
-      } else {
-        flag_store(&flags[t], flag_pack(tag, kFlagAgg, total));
-        prev = lookback(flags, 1, t, tag);
-        flag_store(&flags[t], flag_pack(tag, kFlagInc, prev + total));
-      }
+      if (late) ctrl[kCtrlTimeout] = 1u;
+      if (r.local > 0) flag_store(&flags[t], flag_pack(tag, kFlagInc, prev + total));
       s_excl = prev;
       s_total = total;
       const int32_t n_tiles = (se - sb + T - 1) / T;
@@ -104,8 +99,9 @@ __global__ __launch_bounds__(256) void unique_kernel(
   const uint64_t lt = lanemask_lt();
 #pragma unroll
   for (int k = 0; k < IPT; ++k) {
-    if ((keep_mask[k] >> lane) & 1ull) {
-      const uint32_t pos = s_rows[k * NW + w] + static_cast<uint32_t>(__popcll(keep_mask[k] & lt));
+    const uint64_t m = s_mask[k * NW + w];
+    if ((m >> lane) & 1ull) {
+      const uint32_t pos = s_rows[k * NW + w] + static_cast<uint32_t>(__popcll(m & lt));
       s_keys[pos] = keys[k];
     }
   }
@@ -125,7 +121,7 @@ void launch_ipt(const uint32_t* in, uint32_t* out, const SegView& sv,
 
 }  // namespace
 
-bool scan_config_supported(int ipt) { return ipt == 4 || ipt == 8 || ipt == 16; }
+bool scan_config_supported(int ipt) { return ipt == 8 || ipt == 16 || ipt == 32; }
 
 void launch_unique(const uint32_t* in, uint32_t* out, const SegView& sv,
                    const int32_t* tile_start, int32_t total_tiles,
@@ -133,8 +129,8 @@ void launch_unique(const uint32_t* in, uint32_t* out, const SegView& sv,
                    int32_t* kalloc, const TuneTable& tt, hipStream_t s) {
   if (total_tiles <= 0) return;
   switch (tt.scan_ipt) {
-    case 4: launch_ipt<4>(in, out, sv, tile_start, total_tiles, flags, ctrl, n_unique, kalloc, s); break;
     case 8: launch_ipt<8>(in, out, sv, tile_start, total_tiles, flags, ctrl, n_unique, kalloc, s); break;
+    case 32: launch_ipt<32>(in, out, sv, tile_start, total_tiles, flags, ctrl, n_unique, kalloc, s); break;
     default: launch_ipt<16>(in, out, sv, tile_start, total_tiles, flags, ctrl, n_unique, kalloc, s); break;
   }
 }

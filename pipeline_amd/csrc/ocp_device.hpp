@@ -108,6 +108,31 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* tmp,
   return off + incl - v;
 }
 
+// Exclusive scan, in place, of R words of LDS by one whole wave (each lane
+// takes R/64 consecutive words); returns the total in every lane.
+template <int R>
+__device__ __forceinline__ uint32_t wave_scan_lds(uint32_t* a) {
+  constexpr int PER = (R + kWave - 1) / kWave;
+  const uint32_t lane = lane_id();
+  uint32_t v[PER];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const uint32_t idx = lane * PER + q;
+    v[q] = idx < static_cast<uint32_t>(R) ? a[idx] : 0u;
+    sum += v[q];
+  }
+  const uint32_t incl = wave_incl_scan(sum);
+  uint32_t run = incl - sum;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const uint32_t idx = lane * PER + q;
+    if (idx < static_cast<uint32_t>(R)) a[idx] = run;
+    run += v[q];
+  }
+  return __shfl(incl, kWave - 1, kWave);
+}
+
 // ---------------------------------------------------------------- lookback
 constexpr uint32_t kFlagAgg = 1;  // the tile's own count is published
 constexpr uint32_t kFlagInc = 2;  // the inclusive prefix is published
@@ -124,28 +149,96 @@ __device__ __forceinline__ uint64_t flag_load(const uint64_t* p) {
   return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
 }
-// Sum of the counts of tiles (first, t) of one chain, polling tile k-1,
-// k-2, ... until an inclusive prefix is found. Tiles are handed out by a
-// ticket counter, so every earlier tile of the chain is already resident
-// and will publish; the wait always ends.
-__device__ __forceinline__ uint32_t lookback(const uint64_t* flags,
-                                             uint32_t stride, int32_t t,
-                                             uint32_t tag) {
+// Every wait below is bounded: after kSpinLimit unsuccessful polls the
+// caller gets `timed_out` set, publishes what it has and the kernel runs to
+// its end; the host turns the flag into an error. Tiles are numbered by
+// workgroup id, and workgroups are dispatched in id order on each XCD, so
+// the lowest unfinished tile is always resident and a wait ends long before
+// the limit.
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+// Lookback of one thread per chain (one digit of a OneSweep tile), polling
+// W predecessors per round trip: flags of tiles k-1..k-W are loaded together
+// and consumed in order while ready. `first` is the chain's first tile,
+// which always publishes an inclusive prefix.
+template <int W>
+__device__ __forceinline__ uint32_t lookback_n(const uint64_t* flags,
+                                               uint32_t stride, int32_t t,
+                                               int32_t first, uint32_t tag,
+                                               bool* timed_out) {
   uint32_t excl = 0;
   int32_t k = t - 1;
   const uint32_t want = tag & 0x3fffffffu;
-  while (true) {
-    const uint64_t v = flag_load(flags + static_cast<int64_t>(k) * stride);
-    const uint32_t hi = static_cast<uint32_t>(v >> 32);
-    const uint32_t st = hi & 3u;
-    if ((hi >> 2) != want || st == 0) {
+  for (uint32_t spin = 0; spin < kSpinLimit;) {
+    uint64_t v[W];
+#pragma unroll
+    for (int u = 0; u < W; ++u)
+      v[u] = (k - u >= first) ? flag_load(flags + static_cast<int64_t>(k - u) * stride) : 0ull;
+    int consumed = 0;
+    bool stop = false;
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+      if (stop) continue;
+      const uint32_t hi = static_cast<uint32_t>(v[u] >> 32);
+      const uint32_t st = hi & 3u;
+      if ((hi >> 2) != want || st == 0) {
+        stop = true;
+        continue;
+      }
+      excl += static_cast<uint32_t>(v[u]);
+      if (st == kFlagInc) return excl;
+      ++consumed;
+    }
+    if (consumed == 0) {
       __builtin_amdgcn_s_sleep(1);
+      ++spin;
+    }
+    k -= consumed;
+  }
+  *timed_out = true;
+  return excl;
+}
+
+// Decoupled lookback of a single-value chain by a whole wave: lane l polls
+// tile k-1-l, so 64 predecessors cost one round trip. Every lane of the wave
+// must call it; all lanes return the exclusive prefix of tile t. `first` is
+// the chain's first tile (it publishes an inclusive prefix).
+__device__ __forceinline__ uint32_t wave_lookback(const uint64_t* flags, int32_t t,
+                                                  int32_t first, uint32_t tag,
+                                                  bool* timed_out) {
+  const int32_t lane = static_cast<int32_t>(lane_id());
+  const uint32_t want = tag & 0x3fffffffu;
+  uint32_t excl = 0;
+  int32_t k = t - 1;
+  for (uint32_t spin = 0; spin < kSpinLimit;) {
+    const int32_t kk = k - lane;
+    bool ready = true, inc = true;
+    uint32_t cnt = 0;
+    if (kk >= first) {
+      const uint64_t v = flag_load(flags + kk);
+      const uint32_t hi = static_cast<uint32_t>(v >> 32);
+      const uint32_t st = hi & 3u;
+      ready = (hi >> 2) == want && st != 0;
+      inc = ready && st == kFlagInc;
+      cnt = static_cast<uint32_t>(v);
+    }
+    const uint64_t inc_mask = __ballot(inc);
+    const uint64_t nr_mask = __ballot(!ready);
+    const int f = inc_mask ? __ffsll(static_cast<unsigned long long>(inc_mask)) - 1 : 64;
+    const uint64_t upto = f >= 63 ? ~0ull : ((1ull << (f + 1)) - 1ull);
+    if (nr_mask & upto) {
+      __builtin_amdgcn_s_sleep(1);
+      ++spin;
       continue;
     }
-    excl += static_cast<uint32_t>(v);
-    if (st == kFlagInc) break;
-    --k;
+    uint32_t part = (lane <= f) ? cnt : 0u;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, kWave);
+    excl += part;
+    if (f < 64) return excl;
+    k -= kWave;
   }
+  *timed_out = true;
   return excl;
 }
 

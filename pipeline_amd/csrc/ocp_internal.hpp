@@ -18,6 +18,7 @@ enum CtrlSlot : int {
   kCtrlOctOk = 10,     // 1 when the octree fits the node capacity
   kCtrlLearnFail = 11, // learned sort: buckets that overflowed
   kCtrlLearnBad = 12,  // learned sort: misordered bucket boundaries
+  kCtrlTimeout = 13,   // a lookback wait hit its spin limit (result invalid)
   kCtrlWords = 64,
 };
 
@@ -122,6 +123,18 @@ struct OctArgs {
 void launch_oct_root(OctNode* nodes, const uint32_t* codes,
                      const uint8_t* prefix_n, int32_t n_brt, float mn,
                      float rg, hipStream_t s);
+// mode bits: 1 = make phase (corners, cell sizes, node links), 2 = leaf
+// links, 4 = root corner and cell size, 8 = every slot including the zero
+// gaps (the fused pipeline passes 1|2|4|8).
+// Needs every OctArgs array (the fused build has them all).
+constexpr int kOctMake = 1, kOctLink = 2, kOctRoot = 4, kOctFull = 8;
+void launch_oct_build(const OctArgs& a, const SegView& sv,
+                      const int32_t* tile_start, int32_t total_tiles, int mode,
+                      hipStream_t s);
+
+// ---- stage 7, per-stage entry points (k_octree_stage.hip)
+// The make call of the reference signature has no children arrays
+// (has_leaf_*, left_child may be null with phases/resolve bit 1 only).
 // phase bits: 1 = make writes, 2 = link writes, 4 = leaf-mask AND-NOT
 void launch_oct_scatter(const OctArgs& a, const SegView& sv,
                         const int32_t* tile_start, int32_t total_tiles,

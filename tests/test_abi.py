@@ -1,0 +1,79 @@
+"""C ABI checks that need no GPU: the HIP library loads, exports exactly
+what include/octree_pipeline.h declares, rejects bad batch tables on the
+host, and fails loudly (no CPU fallback) when no device is present."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import pipeline_amd
+from pipeline_amd import clouds
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "octree_pipeline.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    return set(re.findall(r"^(?:int|const char\*)\s+(ocp_\w+)\s*\(", text, re.M))
+
+
+def exported(path):
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True,
+                         text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if " T ocp_" in line}
+
+
+def test_library_exports_every_declared_symbol():
+    lib = pipeline_amd.load_library()
+    names = declared()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(lib, n), n
+    assert exported(pipeline_amd.LIB_PATH) == names
+    assert set(pipeline_amd.SIGNATURES) == names
+
+
+def test_oracle_library_loads():
+    L = oracle.lib()
+    assert L.orc_max_threads() >= 1
+
+
+def test_batch_table_checks_on_host():
+    ok = pipeline_amd.check_batch([0, 5, 5, 9], [0, 1, 2], [1, 1, 1])
+    assert ok == pipeline_amd.OCP_OK
+    assert pipeline_amd.check_batch([1, 5], [0], [1]) == 1           # must start at 0
+    assert pipeline_amd.check_batch([0, 5, 3], [0, 0], [1, 1]) == 1  # decreasing
+    assert pipeline_amd.check_batch([0, 5], [0], [0]) == 1           # zero range
+    assert pipeline_amd.check_batch([0, 5], [float("nan")], [1]) == 1
+    assert pipeline_amd.check_batch([0, (1 << 27) + 1], [0], [1]) == 3  # segment too large
+    assert pipeline_amd.check_batch([0], [], []) == 1               # no segment
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_no_gpu_fails_loudly():
+    with pytest.raises(pipeline_amd.OcpError):
+        pipeline_amd.Context(0)
+
+
+def test_generators_are_seeded_and_clipped():
+    a = clouds.uniform(1000, seed=3)
+    b = clouds.uniform(1000, seed=3)
+    np.testing.assert_array_equal(a, b)
+    assert a.shape == (1000, 4) and (a[:, 3] == 0).all()
+    c = clouds.clustered(1 << 20, seed=3)
+    assert c[:, :3].min() >= 0 and c[:, :3].max() < 1024
+    # a duplicate fraction far above a uniform cloud's after 32-bit Morton
+    # quantisation (uniform: ~0.05% at this size)
+    assert np.unique(oracle.morton(c)).size < 0.97 * c.shape[0]
+    pts, offs, mn, rg, kinds = clouds.batch(8, 10, 1000, seed=1)
+    assert offs[0] == 0 and offs[-1] == pts.shape[0] and (np.diff(offs) > 0).all()
+    for k in range(8):
+        seg = pts[offs[k]:offs[k + 1], :3]
+        q = (seg - mn[k]) / rg[k] * 1024
+        assert q.min() >= 0 and q.max() < 1024
+    assert pipeline_amd.check_batch(offs, mn, rg) == 0
