@@ -98,6 +98,9 @@ def main() -> int:
     ap.add_argument("--workload", default="plain", choices=["plain", "clustered"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 512 MiB device copy (traffic-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -118,6 +121,17 @@ def main() -> int:
 
     ctx = pipeline_amd.Context(0)
     d = ctx.upload(pts)
+    if args.copy_calibration:
+        nb = 512 << 20
+        a_buf = ctx.empty(nb // 4, np.uint32)
+        b_buf = ctx.empty(nb // 4, np.uint32)
+        ctx.copy(b_buf, a_buf, nb)
+        ctx.synchronize()
+        a_buf.free()
+        b_buf.free()
+    tuned = None
+    if not args.no_autotune:
+        tuned = ctx.autotune(d, n=n, reps=3)
     # Eager builds with per-kernel HIP events.
     ctx.set_stage_timing(True)
     kms: dict[str, list[float]] = {}
@@ -162,7 +176,8 @@ def main() -> int:
         "dtype": "u32",
         "data": "synthetic",
         "config": {"workload": wl, "n_points": n, "n_unique": u, "n_oct": k,
-                   "seed": 114514, "hipgraph": True, "tune": ctx.get_tune()},
+                   "seed": 114514, "hipgraph": True, "tune": ctx.get_tune(),
+                   "autotuned": tuned is not None},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

@@ -63,6 +63,9 @@ int ocp_free(ocp_ctx ctx, void* ptr);
 int ocp_memcpy_h2d(ocp_ctx ctx, void* dst, const void* src, size_t bytes);
 int ocp_memcpy_d2h(ocp_ctx ctx, void* dst, const void* src, size_t bytes);
 int ocp_memset(ocp_ctx ctx, void* dst, int value, size_t bytes);
+/* Device-to-device copy kernel on the context stream (16-byte aligned
+ * pointers, size a multiple of 16). */
+int ocp_copy(ocp_ctx ctx, void* dst, const void* src, size_t bytes);
 
 /* ------------------------------------------------------------ stage 1 */
 /* Replaces gpu::k_ComputeMortonCode (include/cuda/kernels/01_morton.cuh:7-11)

@@ -79,6 +79,7 @@ SIGNATURES = {
     "ocp_memcpy_h2d": (_INT, [_P, _P, _P, _SZ]),
     "ocp_memcpy_d2h": (_INT, [_P, _P, _P, _SZ]),
     "ocp_memset": (_INT, [_P, _P, _INT, _SZ]),
+    "ocp_copy": (_INT, [_P, _P, _P, _SZ]),
     "ocp_k_ComputeMortonCode": (_INT, [_P, _P, _P, _SZ, _F, _F]),
     "ocp_k_ComputeMortonCode3": (_INT, [_P, _P, _P, _SZ, _F, _F]),
     "ocp_k_GlobalHistogram": (_INT, [_P, _P, _P, _U32]),
@@ -242,6 +243,10 @@ class Context:
 
     def synchronize(self):
         self._check(self.lib.ocp_synchronize(self.h))
+
+    def copy(self, dst: DeviceArray, src: DeviceArray, nbytes: int):
+        """Device-to-device copy kernel (16-byte multiples)."""
+        self._check(self.lib.ocp_copy(self.h, dst.ptr, src.ptr, nbytes))
 
     def close(self):
         if getattr(self, "h", None):

@@ -608,6 +608,17 @@ int ocp_memcpy_d2h(ocp_ctx c, void* dst, const void* src, size_t bytes) {
   OCP_HIP(c, hipStreamSynchronize(c->stream));
   return OCP_OK;
 }
+int ocp_copy(ocp_ctx c, void* dst, const void* src, size_t bytes) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (!bytes) return OCP_OK;
+  if (!dst || !src || (bytes & 15u) || (reinterpret_cast<uintptr_t>(dst) & 15u) ||
+      (reinterpret_cast<uintptr_t>(src) & 15u))
+    return fail(c, OCP_ERR_INVALID, "ocp_copy needs 16-byte aligned pointers and size");
+  launch_copy(dst, src, static_cast<int64_t>(bytes), c->stream);
+  OCP_CHECK_LAUNCH(c);
+  return OCP_OK;
+}
+
 int ocp_memset(ocp_ctx c, void* dst, int value, size_t bytes) {
   if (check_ctx(c)) return OCP_ERR_INVALID;
   if (!bytes) return OCP_OK;

@@ -68,7 +68,26 @@ __global__ __launch_bounds__(256) void zero_nodes_kernel(OctNode* nodes,
     p[i] = make_uint4(0, 0, 0, 0);
 }
 
+// Plain 16-byte-per-lane copy (device to device); also the known-size
+// reference kernel that HBM traffic counters are calibrated against.
+__global__ __launch_bounds__(256) void copy_kernel(uint4* __restrict__ dst,
+                                                   const uint4* __restrict__ src,
+                                                   int64_t n16) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+       i < n16; i += stride)
+    dst[i] = src[i];
+}
+
 }  // namespace
+
+void launch_copy(void* dst, const void* src, int64_t bytes, hipStream_t s) {
+  const int64_t n16 = bytes / 16;
+  if (n16 <= 0) return;
+  const int64_t blocks = std::min<int64_t>((n16 + 255) / 256, 8192);
+  copy_kernel<<<static_cast<int>(blocks), 256, 0, s>>>(
+      static_cast<uint4*>(dst), static_cast<const uint4*>(src), n16);
+}
 
 void launch_fill_u32(uint32_t* p, uint32_t value, int64_t n_words,
                      hipStream_t s) {

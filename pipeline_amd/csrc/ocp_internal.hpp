@@ -33,7 +33,7 @@ enum Stage : uint32_t {
 // Kernel configuration chosen by the autotuner (or the defaults).
 struct TuneTable {
   int sort_waves = 8;      // OneSweep workgroup = sort_waves * 64 threads
-  int sort_kpt = 15;       // keys per thread in OneSweep
+  int sort_kpt = 12;       // keys per thread in OneSweep
   int morton_ppt = 16;     // points per thread in the Morton kernel
   int morton_grid = 1024;  // persistent Morton workgroups
   int scan_ipt = 16;       // items per thread in unique / edge-scan tiles
@@ -49,6 +49,8 @@ bool scan_config_supported(int ipt);
 bool morton_config_supported(int ppt);
 
 // ---- utility kernels (k_util.hip)
+// bytes must be a multiple of 16 and both pointers 16-byte aligned.
+void launch_copy(void* dst, const void* src, int64_t bytes, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t value, int64_t n_words,
                      hipStream_t s);
 // Zero tickets, bump epoch; optionally zero n_hist words of histogram.

@@ -1,0 +1,153 @@
+// One stable OneSweep digit pass, generic in how a key maps to its 8-bit
+// digit (a radix digit for the LSD sort, a learned-CDF bucket for the
+// learned sort). Shared by k_onesweep.hip and k_learned.hip.
+//
+//   1. Tile = workgroup id. Workgroups are dispatched in id order on each
+//      XCD, so the lowest unfinished tile is always resident and the
+//      lookback makes progress without a shared ticket counter (a single
+//      device-scope counter serialises at one atomic per tile).
+//   2. Keys are loaded wave-contiguously (lane-strided, coalesced) and
+//      ranked per wave with a 64-lane multisplit: eight ballots build the
+//      mask of lanes holding the same digit; popcount below the lane gives a
+//      stable rank; the lowest lane bumps the wave's LDS digit counter.
+//   3. The tile's 256 digit counts are published at once (aggregate, or
+//      inclusive for a segment's first tile), then each of 256 threads looks
+//      back along its digit's chain, 16 tiles per round trip, with
+//      agent-scope loads, and publishes the inclusive prefix.
+//   4. Keys (and their digits) are scattered into LDS in digit order, then
+//      written out with consecutive threads on consecutive LDS slots, so
+//      equal-digit runs go to consecutive HBM addresses.
+// Segments sort independently: each has its own digit histogram and its own
+// lookback chain, and no tile holds keys of two segments.
+#pragma once
+
+#include "ocp_internal.hpp"
+
+namespace ocp {
+
+struct RadixDigit {
+  uint32_t shift;
+  __device__ __forceinline__ uint32_t operator()(uint32_t key) const {
+    return (key >> shift) & 255u;
+  }
+};
+
+// hist: per segment 1024 words; this pass reads words [slot*256, slot*256+256)
+// of its segment. tiles_dev (optional): the number of real tiles, on device,
+// when the grid is sized for an upper bound.
+template <int WAVES, int KPT, class Digit>
+__global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
+    const uint32_t* __restrict__ in, uint32_t* __restrict__ out, SegView sv,
+    const int32_t* __restrict__ tile_start, const int32_t* __restrict__ tiles_dev,
+    const uint32_t* __restrict__ hist, int hist_slot, uint64_t* __restrict__ flags,
+    uint32_t* __restrict__ ctrl, uint32_t stage, Digit dig) {
+  constexpr int THREADS = WAVES * 64;
+  constexpr int T = THREADS * KPT;
+  static_assert(THREADS >= 256, "one thread per digit is required");
+  __shared__ uint32_t s_keys[T];
+  __shared__ uint8_t s_dig[T];
+  __shared__ uint32_t s_whist[WAVES * 256];
+  __shared__ uint32_t s_dstart[256];
+  __shared__ int32_t s_dst[256];
+  __shared__ uint32_t s_tmp[WAVES];
+
+  const int32_t t = blockIdx.x;
+  if (tiles_dev && t >= *tiles_dev) return;
+  const int tid = threadIdx.x;
+  const uint32_t w = wave_id();
+  const uint32_t lane = lane_id();
+  for (int i = tid; i < WAVES * 256; i += THREADS) s_whist[i] = 0;
+  __syncthreads();
+  const uint32_t tag = ctrl[kCtrlEpoch] * 8u + stage;
+  const TileRef r = find_tile(tile_start, sv.n_seg, t);
+  const int32_t sb = seg_begin(sv, r.seg);
+  const int32_t base = sb + r.local * T;
+  const int32_t cnt = min(T, seg_end(sv, r.seg) - base);
+
+  uint32_t keys[KPT];
+  uint32_t dg[KPT];
+  uint32_t off[KPT];
+  const uint32_t* src = in + base;
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
+    keys[k] = idx < cnt ? src[idx] : 0xffffffffu;
+  }
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
+    dg[k] = idx < cnt ? dig(keys[k]) : 255u;  // padding keys rank last
+  }
+
+  // Wave-level multisplit. The read of the digit counter by every lane of a
+  // group happens in the same LDS instruction, before the leader's update.
+  uint32_t* wh = s_whist + w * 256;
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    const uint32_t d = dg[k];
+    uint64_t m = ~0ull;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bal = __ballot(bit);
+      m &= bit ? bal : ~bal;
+    }
+    const uint32_t rank = __popcll(m & lt);
+    const uint32_t old = wh[d];
+    if (rank == 0) wh[d] = old + __popcll(m);
+    off[k] = old + rank;
+  }
+  __syncthreads();
+
+  // Per-digit: exclusive prefix over waves, tile count, publish.
+  uint32_t tile_cnt = 0;
+  if (tid < 256) {
+    uint32_t run = 0;
+#pragma unroll
+    for (int ww = 0; ww < WAVES; ++ww) {
+      const uint32_t c = s_whist[ww * 256 + tid];
+      s_whist[ww * 256 + tid] = run;
+      run += c;
+    }
+    if (tid == 255) run -= static_cast<uint32_t>(T - cnt);  // padding keys
+    tile_cnt = run;
+    flag_store(&flags[static_cast<int64_t>(t) * 256 + tid],
+               flag_pack(tag, r.local == 0 ? kFlagInc : kFlagAgg, run));
+  }
+  uint32_t total;
+  const uint32_t dstart = block_excl_scan<THREADS>(tile_cnt, s_tmp, &total);
+  const uint32_t hv =
+      tid < 256 ? hist[static_cast<int64_t>(r.seg) * 1024 + hist_slot * 256 + tid] : 0u;
+  const uint32_t hex = block_excl_scan<THREADS>(hv, s_tmp, &total);
+  if (tid < 256) {
+    s_dstart[tid] = dstart;
+    uint32_t prev = 0;
+    if (r.local > 0) {
+      bool late = false;
+      prev = lookback_n<16>(flags + tid, 256, t, t - r.local, tag, &late);
+      if (late) ctrl[kCtrlTimeout] = 1u;
+      flag_store(&flags[static_cast<int64_t>(t) * 256 + tid],
+                 flag_pack(tag, kFlagInc, prev + tile_cnt));
+    }
+    s_dst[tid] = sb + static_cast<int32_t>(hex + prev) - static_cast<int32_t>(dstart);
+  }
+  __syncthreads();
+
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    const uint32_t d = dg[k];
+    const uint32_t pos = s_dstart[d] + wh[d] + off[k];
+    s_keys[pos] = keys[k];
+    s_dig[pos] = static_cast<uint8_t>(d);
+  }
+  __syncthreads();
+
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    const int32_t p = k * THREADS + tid;
+    if (p < cnt) out[s_dst[s_dig[p]] + p] = s_keys[p];
+  }
+}
+
+}  // namespace ocp
