@@ -241,6 +241,67 @@ int ocp_autotune(ocp_ctx ctx, const float* points, int float3, int64_t n,
                  const float* seg_range, int32_t n_seg, int reps,
                  ocp_tune* chosen);
 
+/* ------------------------------------------------------------ learned sort
+ * No counterpart in the reference, which sorts with OneSweep only
+ * (apps/gpu_demo/main.cu:63-125). A drop-in for ocp_k_SortKeys when the key
+ * distribution is known from earlier frames: a small FP32 CDF model predicts
+ * 16384 bucket boundaries, keys are scattered into buckets and each bucket is
+ * sorted in LDS; oversized buckets are sorted by OneSweep and every bucket
+ * boundary is checked, so the output bytes equal ocp_k_SortKeys' whatever
+ * the model is.
+ *
+ * Model: root r(x) = b2 + sum_j w2[j] relu(w1[j] x + b1[j]) (16 units),
+ * x = (key - kmin) / (kmax - kmin + 1), leaf l = floor(4096 r) clamped,
+ * y = clamp(a[l] + b[l] (key - K_l), lo[l], lo[l+1]) with K_l the first key
+ * of leaf l, bucket = floor(16384 y) clamped. Parameter layout (FP32):
+ * w1[16] b1[16] w2[16] b2 | lo[4097] | a[4096] | b[4096]. */
+#define OCP_CDF_PARAMS 12338
+#define OCP_CDF_ROOT_PARAMS 49
+#define OCP_CDF_BUCKETS 16384
+#define OCP_CDF_SAMPLE_CAP (1 << 22)
+
+typedef struct ocp_learned_stats {
+  int64_t n;                  /* keys sorted */
+  int32_t n_buckets_nonempty;
+  int32_t max_bucket;         /* keys in the largest bucket */
+  int32_t n_big;              /* buckets above 16384 keys (OneSweep fallback) */
+  int32_t misordered;         /* bucket boundaries found out of order */
+  int64_t keys_big;           /* keys in the fallback buckets */
+  int32_t full_resort;        /* 1 if the whole array was re-sorted by OneSweep */
+  int32_t reserved;
+} ocp_learned_stats;
+
+/* The linear CDF over [kmin, kmax] (identity root, uniform leaves). */
+int ocp_cdf_reset(ocp_ctx ctx, uint32_t kmin, uint32_t kmax);
+/* Append keys[0], keys[stride], ... (device pointer) to the sample ring of
+ * OCP_CDF_SAMPLE_CAP keys; the oldest samples are overwritten. */
+int ocp_cdf_add_samples(ocp_ctx ctx, const uint32_t* keys, int64_t n, int32_t stride);
+int ocp_cdf_clear_samples(ocp_ctx ctx);
+int ocp_cdf_sample_count(ocp_ctx ctx, int64_t* count);
+/* Sort the samples, train the root from the identity by `steps` SGD steps
+ * (batch 1024, momentum, deterministic for a seed), fit the leaves by least
+ * squares and compile the bucket boundaries. Writes the last batch loss. */
+int ocp_cdf_train(ocp_ctx ctx, int32_t steps, float lr, float momentum, uint32_t seed,
+                  float* loss_out);
+/* Host copies of the parameters (OCP_CDF_PARAMS floats) and key range. */
+int ocp_cdf_get(ocp_ctx ctx, float* params, uint32_t* kmin, uint32_t* kmax);
+int ocp_cdf_set(ocp_ctx ctx, const float* params, uint32_t kmin, uint32_t kmax);
+/* 32-byte header ("OCPCDF1\0", u32 version 1, u32 OCP_CDF_PARAMS, u32 kmin,
+ * u32 kmax, u32 16, u32 4096) + the parameters as little-endian FP32. */
+int ocp_cdf_save(ocp_ctx ctx, const char* path);
+int ocp_cdf_load(ocp_ctx ctx, const char* path);
+/* Mean gradient of 0.5 (r(x) - t)^2 over n device pairs (xs, ts) with
+ * respect to the 49 root parameters, into a host array. */
+int ocp_cdf_grad(ocp_ctx ctx, const float* xs, const float* ts, int32_t n, float* grad);
+/* Bucket of each key (device arrays): mode 0 evaluates the model, mode 1
+ * the compiled boundaries (the two agree). */
+int ocp_cdf_buckets(ocp_ctx ctx, const uint32_t* keys, uint32_t n, int mode, uint32_t* q);
+/* Host copy of the 16384 boundaries: bucket(key) = #{f >= 1 : fb[f] <= key}. */
+int ocp_cdf_boundaries(ocp_ctx ctx, uint32_t* fb);
+/* Sort n keys in place (keys_alt: n words of scratch). Synchronous. */
+int ocp_sort_learned(ocp_ctx ctx, uint32_t* keys, uint32_t* keys_alt, uint32_t n,
+                     ocp_learned_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,21 @@ class Tune(ctypes.Structure):
         return {f: int(getattr(self, f)) for f, _ in self._fields_}
 
 
+class LearnedStats(ctypes.Structure):
+    _fields_ = [("n", _I64), ("n_buckets_nonempty", _I32), ("max_bucket", _I32),
+                ("n_big", _I32), ("misordered", _I32), ("keys_big", _I64),
+                ("full_resort", _I32), ("reserved", _I32)]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_ if f != "reserved"}
+
+
+CDF_PARAMS = 12338
+CDF_ROOT_PARAMS = 49
+CDF_BUCKETS = 16384
+CDF_SAMPLE_CAP = 1 << 22
+
+
 # Every function the C ABI exports, with its ctypes signature. The CPU test
 # suite checks that the built library exports exactly these.
 SIGNATURES = {
@@ -105,6 +120,19 @@ SIGNATURES = {
     "ocp_get_tune": (_INT, [_P, ctypes.POINTER(Tune)]),
     "ocp_set_tune": (_INT, [_P, ctypes.POINTER(Tune)]),
     "ocp_autotune": (_INT, [_P, _P, _INT, _I64, _P, _P, _P, _I32, _INT, ctypes.POINTER(Tune)]),
+    "ocp_cdf_reset": (_INT, [_P, _U32, _U32]),
+    "ocp_cdf_add_samples": (_INT, [_P, _P, _I64, _I32]),
+    "ocp_cdf_clear_samples": (_INT, [_P]),
+    "ocp_cdf_sample_count": (_INT, [_P, ctypes.POINTER(_I64)]),
+    "ocp_cdf_train": (_INT, [_P, _I32, _F, _F, _U32, ctypes.POINTER(_F)]),
+    "ocp_cdf_get": (_INT, [_P, _P, ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
+    "ocp_cdf_set": (_INT, [_P, _P, _U32, _U32]),
+    "ocp_cdf_save": (_INT, [_P, ctypes.c_char_p]),
+    "ocp_cdf_load": (_INT, [_P, ctypes.c_char_p]),
+    "ocp_cdf_grad": (_INT, [_P, _P, _P, _I32, _P]),
+    "ocp_cdf_buckets": (_INT, [_P, _P, _U32, _INT, _P]),
+    "ocp_cdf_boundaries": (_INT, [_P, _P]),
+    "ocp_sort_learned": (_INT, [_P, _P, _P, _U32, ctypes.POINTER(LearnedStats)]),
 }
 
 _lib = None
@@ -459,3 +487,79 @@ class Context:
                                               _ptr(offs), _ptr(mn), _ptr(rg), offs.size - 1,
                                               reps, ctypes.byref(t)))
         return t.as_dict()
+
+    # ---- learned-CDF bucket sort
+    def cdf_reset(self, kmin: int = 0, kmax: int = (1 << 30) - 1):
+        """Linear CDF over [kmin, kmax] (the untrained model)."""
+        self._check(self.lib.ocp_cdf_reset(self.h, int(kmin), int(kmax)))
+
+    def cdf_add_samples(self, keys: DeviceArray, n: int | None = None, stride: int = 100):
+        n = keys.size if n is None else n
+        self._check(self.lib.ocp_cdf_add_samples(self.h, keys.ptr, int(n), int(stride)))
+
+    def cdf_clear_samples(self):
+        self._check(self.lib.ocp_cdf_clear_samples(self.h))
+
+    def cdf_sample_count(self) -> int:
+        v = _I64()
+        self._check(self.lib.ocp_cdf_sample_count(self.h, ctypes.byref(v)))
+        return int(v.value)
+
+    def cdf_train(self, steps: int = 2000, lr: float = 1.0, momentum: float = 0.9,
+                  seed: int = 1) -> float:
+        loss = _F()
+        self._check(self.lib.ocp_cdf_train(self.h, int(steps), float(lr), float(momentum),
+                                           int(seed), ctypes.byref(loss)))
+        return float(loss.value)
+
+    def cdf_get(self):
+        """(params float32[CDF_PARAMS], kmin, kmax)"""
+        p = np.empty(CDF_PARAMS, np.float32)
+        lo, hi = _U32(), _U32()
+        self._check(self.lib.ocp_cdf_get(self.h, _ptr(p), ctypes.byref(lo), ctypes.byref(hi)))
+        return p, int(lo.value), int(hi.value)
+
+    def cdf_set(self, params: np.ndarray, kmin: int, kmax: int):
+        p = np.ascontiguousarray(params, np.float32)
+        if p.size != CDF_PARAMS:
+            raise ValueError(f"expected {CDF_PARAMS} parameters, got {p.size}")
+        self._check(self.lib.ocp_cdf_set(self.h, _ptr(p), int(kmin), int(kmax)))
+
+    def cdf_save(self, path: str):
+        self._check(self.lib.ocp_cdf_save(self.h, os.fsencode(path)))
+
+    def cdf_load(self, path: str):
+        self._check(self.lib.ocp_cdf_load(self.h, os.fsencode(path)))
+
+    def cdf_grad(self, xs: np.ndarray, ts: np.ndarray) -> np.ndarray:
+        """Mean gradient of 0.5 (r(x) - t)^2 over the current root parameters."""
+        dx = self.upload(np.ascontiguousarray(xs, np.float32))
+        dt = self.upload(np.ascontiguousarray(ts, np.float32))
+        g = np.empty(CDF_ROOT_PARAMS, np.float32)
+        try:
+            self._check(self.lib.ocp_cdf_grad(self.h, dx.ptr, dt.ptr, int(dx.size), _ptr(g)))
+        finally:
+            dx.free()
+            dt.free()
+        return g
+
+    def cdf_buckets(self, keys: DeviceArray, n: int | None = None, mode: int = 1) -> np.ndarray:
+        n = keys.size if n is None else n
+        q = self.empty(max(n, 1), np.uint32)
+        try:
+            self._check(self.lib.ocp_cdf_buckets(self.h, keys.ptr, int(n), int(mode), q.ptr))
+            return q.to_host(n)
+        finally:
+            q.free()
+
+    def cdf_boundaries(self) -> np.ndarray:
+        fb = np.empty(CDF_BUCKETS, np.uint32)
+        self._check(self.lib.ocp_cdf_boundaries(self.h, _ptr(fb)))
+        return fb
+
+    def sort_learned(self, keys: DeviceArray, keys_alt: DeviceArray, n: int | None = None) -> dict:
+        n = keys.size if n is None else n
+        st = LearnedStats()
+        self._check(self.lib.ocp_sort_learned(self.h, keys.ptr, keys_alt.ptr, int(n),
+                                              ctypes.byref(st)))
+        return st.as_dict()

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/octree_pipeline.h"
+#include "learned.hpp"
 #include "ocp_internal.hpp"
 
 using namespace ocp;
@@ -83,6 +84,28 @@ struct LastBuild {
   std::vector<float> seg_min, seg_range;
 };
 
+// Learned sort: the model, its compiled tables, the key-sample ring and
+// the per-sort tables (allocated on first use).
+struct LearnState {
+  float* model = nullptr;     // kCdfParams
+  uint32_t* meta = nullptr;   // [kmin, kmax, 0, 0]
+  uint32_t* kleaf = nullptr;  // kCdfLeaves + 1
+  uint32_t* fb = nullptr;     // kLearnBuckets
+  float* loss = nullptr;
+  bool compiled = false;
+  uint32_t* ring = nullptr;   // sample ring
+  uint32_t* samp_a = nullptr;
+  uint32_t* samp_b = nullptr;
+  int64_t ring_fill = 0, ring_pos = 0;
+  // per-sort tables
+  int64_t n_cap = 0;
+  LearnTables tb{};
+  uint32_t* big_hist = nullptr;  // big_cap * 1024
+  int32_t* h_info = nullptr;     // pinned host copy of tb.info
+  hipEvent_t ev_info = nullptr;
+};
+constexpr int64_t kSampleCap = int64_t(1) << 22;
+
 }  // namespace
 
 struct ocp_context {
@@ -110,6 +133,7 @@ struct ocp_context {
   hipEvent_t kev[2 * kKernelCount] = {};
   bool ev_recorded = false;
   LastBuild last;
+  LearnState learn;
 };
 
 namespace {
@@ -523,6 +547,19 @@ int settle(ocp_ctx c) {
 
 int check_ctx(ocp_ctx c) { return c ? OCP_OK : OCP_ERR_INVALID; }
 
+void free_learn(LearnState& L) {
+  dev_free(L.model); dev_free(L.meta); dev_free(L.kleaf); dev_free(L.fb); dev_free(L.loss);
+  dev_free(L.ring); dev_free(L.samp_a); dev_free(L.samp_b);
+  LearnTables& t = L.tb;
+  dev_free(t.rows); dev_free(t.hist); dev_free(t.bstart); dev_free(t.hist_a);
+  dev_free(t.hist_b); dev_free(t.seg_start); dev_free(t.tile_start); dev_free(t.tiles_b);
+  dev_free(t.big_start); dev_free(t.big_end); dev_free(t.big_tiles); dev_free(t.info);
+  dev_free(L.big_hist);
+  if (L.h_info) (void)hipHostFree(L.h_info);
+  if (L.ev_info) (void)hipEventDestroy(L.ev_info);
+  L = LearnState{};
+}
+
 }  // namespace
 
 // ======================================================================
@@ -570,6 +607,7 @@ int ocp_destroy(ocp_ctx c) {
     if (e) (void)hipEventDestroy(e);
   for (auto& e : c->kev)
     if (e) (void)hipEventDestroy(e);
+  free_learn(c->learn);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return OCP_OK;
@@ -1051,6 +1089,330 @@ int ocp_autotune(ocp_ctx c, const float* points, int float3, int64_t n,
   }
   if (int r = ocp_set_tune(c, &best)) return r;
   if (chosen) *chosen = best;
+  return OCP_OK;
+}
+
+// ---------------------------------------------------------------- learned sort
+static int learn_model_alloc(ocp_ctx c) {
+  LearnState& L = c->learn;
+  if (L.model) return OCP_OK;
+  int r = 0;
+  if ((r = dev_alloc(c, &L.model, kCdfParams)) || (r = dev_alloc(c, &L.meta, 4)) ||
+      (r = dev_alloc(c, &L.kleaf, kCdfLeaves + 1)) || (r = dev_alloc(c, &L.fb, kLearnBuckets)) ||
+      (r = dev_alloc(c, &L.loss, 1)))
+    return r;
+  OCP_HIP(c, hipMemsetAsync(L.model, 0, sizeof(float) * kCdfParams, c->stream));
+  OCP_HIP(c, hipMemsetAsync(L.meta, 0, sizeof(uint32_t) * 4, c->stream));
+  return OCP_OK;
+}
+
+static int learn_tables_alloc(ocp_ctx c, int64_t n) {
+  LearnState& L = c->learn;
+  LearnTables& t = L.tb;
+  if (!t.rows) {
+    int r = 0;
+    if ((r = dev_alloc(c, &t.rows, int64_t(kLearnHistGrid) * kLearnBuckets)) ||
+        (r = dev_alloc(c, &t.hist, kLearnBuckets)) ||
+        (r = dev_alloc(c, &t.bstart, kLearnBuckets + 1)) || (r = dev_alloc(c, &t.hist_a, 1024)) ||
+        (r = dev_alloc(c, &t.hist_b, int64_t(kLearnCoarse) * 1024)) ||
+        (r = dev_alloc(c, &t.seg_start, kLearnCoarse + 1)) ||
+        (r = dev_alloc(c, &t.tile_start, kLearnCoarse + 1)) || (r = dev_alloc(c, &t.tiles_b, 1)) ||
+        (r = dev_alloc(c, &t.info, 8)))
+      return r;
+    // The second scatter reads 256 counts per segment and only 64 are ever
+    // written; the rest stay zero.
+    launch_fill_u32(t.hist_a, 0u, 1024, c->stream);
+    launch_fill_u32(t.hist_b, 0u, int64_t(kLearnCoarse) * 1024, c->stream);
+    OCP_HIP(c, hipHostMalloc(reinterpret_cast<void**>(&L.h_info), sizeof(int32_t) * 8,
+                             hipHostMallocDefault));
+    OCP_HIP(c, hipEventCreateWithFlags(&L.ev_info, hipEventDisableTiming));
+  }
+  if (n > L.n_cap) {
+    const int64_t cap = std::max<int64_t>(n, 1 << 20);
+    const int32_t big_cap = static_cast<int32_t>(cap / (kLearnBigCap + 1) + 1);
+    dev_free(t.big_start); dev_free(t.big_end); dev_free(t.big_tiles); dev_free(L.big_hist);
+    int r = 0;
+    if ((r = dev_alloc(c, &t.big_start, big_cap)) || (r = dev_alloc(c, &t.big_end, big_cap)) ||
+        (r = dev_alloc(c, &t.big_tiles, int64_t(big_cap) + 1)) ||
+        (r = dev_alloc(c, &L.big_hist, int64_t(big_cap) * 1024)))
+      return r;
+    t.big_cap = big_cap;
+    L.n_cap = cap;
+  }
+  t.fb = L.fb;
+  return OCP_OK;
+}
+
+// Compile the current parameters: leaf starts from the root, then the
+// bucket boundaries.
+static int learn_compile(ocp_ctx c, bool fit_from_samples, bool linear_leaves) {
+  LearnState& L = c->learn;
+  launch_lleaf_starts(L.model, L.meta, L.kleaf, c->stream);
+  if (fit_from_samples)
+    launch_lfit_leaves(L.samp_a, static_cast<int32_t>(L.ring_fill), L.kleaf, L.model, c->stream);
+  if (linear_leaves) launch_llinear_leaves(L.kleaf, L.meta, L.model, c->stream);
+  launch_lboundaries(L.model, L.meta, L.kleaf, L.fb, c->stream);
+  OCP_CHECK_LAUNCH(c);
+  L.compiled = true;
+  return OCP_OK;
+}
+
+int ocp_cdf_reset(ocp_ctx c, uint32_t kmin, uint32_t kmax) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (kmax < kmin) return fail(c, OCP_ERR_INVALID, "kmax < kmin");
+  if (int r = learn_model_alloc(c)) return r;
+  const uint32_t meta[4] = {kmin, kmax, 0u, 0u};
+  OCP_HIP(c, hipMemcpyAsync(c->learn.meta, meta, sizeof(meta), hipMemcpyHostToDevice, c->stream));
+  launch_llinear(c->learn.model, c->stream);
+  if (int r = learn_compile(c, false, true)) return r;
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  return OCP_OK;
+}
+
+int ocp_cdf_add_samples(ocp_ctx c, const uint32_t* keys, int64_t n, int32_t stride) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (n < 0 || stride < 1) return fail(c, OCP_ERR_INVALID, "n < 0 or stride < 1");
+  if (n > 0 && !keys) return fail(c, OCP_ERR_INVALID, "null pointer");
+  LearnState& L = c->learn;
+  if (!L.ring) {
+    int r = 0;
+    if ((r = dev_alloc(c, &L.ring, kSampleCap)) || (r = dev_alloc(c, &L.samp_a, kSampleCap)) ||
+        (r = dev_alloc(c, &L.samp_b, kSampleCap)))
+      return r;
+  }
+  int64_t count = (n + stride - 1) / stride;
+  const uint32_t* src = keys;
+  if (count > kSampleCap) {  // only the last kSampleCap samples would survive
+    src = keys + (count - kSampleCap) * static_cast<int64_t>(stride);
+    count = kSampleCap;
+  }
+  launch_lsample(src, n, stride, L.ring, kSampleCap, L.ring_pos, count, c->stream);
+  OCP_CHECK_LAUNCH(c);
+  L.ring_pos = (L.ring_pos + count) % kSampleCap;
+  L.ring_fill = std::min<int64_t>(kSampleCap, L.ring_fill + count);
+  return OCP_OK;
+}
+
+int ocp_cdf_clear_samples(ocp_ctx c) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  c->learn.ring_fill = 0;
+  c->learn.ring_pos = 0;
+  return OCP_OK;
+}
+
+int ocp_cdf_sample_count(ocp_ctx c, int64_t* count) {
+  if (check_ctx(c) || !count) return OCP_ERR_INVALID;
+  *count = c->learn.ring_fill;
+  return OCP_OK;
+}
+
+int ocp_cdf_train(ocp_ctx c, int32_t steps, float lr, float momentum, uint32_t seed,
+                  float* loss_out) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  LearnState& L = c->learn;
+  if (L.ring_fill < 2) return fail(c, OCP_ERR_STATE, "fewer than 2 key samples: add samples first");
+  if (steps < 0) return fail(c, OCP_ERR_INVALID, "steps < 0");
+  if (int r = learn_model_alloc(c)) return r;
+  const int32_t ns = static_cast<int32_t>(L.ring_fill);
+  OCP_HIP(c, hipMemcpyAsync(L.samp_a, L.ring, sizeof(uint32_t) * ns, hipMemcpyDeviceToDevice,
+                            c->stream));
+  if (int r = ocp_k_SortKeys(c, L.samp_a, L.samp_b, static_cast<uint32_t>(ns))) return r;
+  launch_lmeta(L.samp_a, ns, L.meta, c->stream);
+  launch_llinear(L.model, c->stream);
+  launch_ltrain_root(L.samp_a, ns, L.meta, steps, lr, momentum, seed, L.model, L.loss, c->stream);
+  if (int r = learn_compile(c, true, false)) return r;
+  float loss = 0.0f;
+  OCP_HIP(c, hipMemcpyAsync(&loss, L.loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  if (loss_out) *loss_out = steps > 0 ? loss : 0.0f;
+  return OCP_OK;
+}
+
+int ocp_cdf_get(ocp_ctx c, float* params, uint32_t* kmin, uint32_t* kmax) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  LearnState& L = c->learn;
+  if (!L.compiled) return fail(c, OCP_ERR_STATE, "no CDF model: train, set, reset or load one");
+  uint32_t meta[4];
+  OCP_HIP(c, hipMemcpyAsync(meta, L.meta, sizeof(meta), hipMemcpyDeviceToHost, c->stream));
+  if (params)
+    OCP_HIP(c, hipMemcpyAsync(params, L.model, sizeof(float) * kCdfParams, hipMemcpyDeviceToHost,
+                              c->stream));
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  if (kmin) *kmin = meta[0];
+  if (kmax) *kmax = meta[1];
+  return OCP_OK;
+}
+
+int ocp_cdf_set(ocp_ctx c, const float* params, uint32_t kmin, uint32_t kmax) {
+  if (check_ctx(c) || !params) return OCP_ERR_INVALID;
+  if (kmax < kmin) return fail(c, OCP_ERR_INVALID, "kmax < kmin");
+  if (int r = learn_model_alloc(c)) return r;
+  LearnState& L = c->learn;
+  const uint32_t meta[4] = {kmin, kmax, 0u, 0u};
+  OCP_HIP(c, hipMemcpyAsync(L.meta, meta, sizeof(meta), hipMemcpyHostToDevice, c->stream));
+  OCP_HIP(c, hipMemcpyAsync(L.model, params, sizeof(float) * kCdfParams, hipMemcpyHostToDevice,
+                            c->stream));
+  if (int r = learn_compile(c, false, false)) return r;
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  return OCP_OK;
+}
+
+// File: 32-byte header (magic "OCPCDF1", u32 version 1, u32 parameter
+// count, u32 kmin, u32 kmax, u32 hidden units, u32 leaves), then the
+// parameters as little-endian FP32.
+static const char kCdfMagic[8] = {'O', 'C', 'P', 'C', 'D', 'F', '1', '\0'};
+
+int ocp_cdf_save(ocp_ctx c, const char* path) {
+  if (check_ctx(c) || !path) return OCP_ERR_INVALID;
+  std::vector<float> p(kCdfParams);
+  uint32_t kmin = 0, kmax = 0;
+  if (int r = ocp_cdf_get(c, p.data(), &kmin, &kmax)) return r;
+  const uint32_t hdr[6] = {1u, static_cast<uint32_t>(kCdfParams), kmin, kmax,
+                           static_cast<uint32_t>(kCdfHidden), static_cast<uint32_t>(kCdfLeaves)};
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(c, OCP_ERR_INVALID, std::string("cannot open ") + path);
+  const bool ok = std::fwrite(kCdfMagic, 1, 8, f) == 8 && std::fwrite(hdr, 4, 6, f) == 6 &&
+                  std::fwrite(p.data(), 4, p.size(), f) == p.size();
+  const bool closed = std::fclose(f) == 0;
+  if (!ok || !closed) return fail(c, OCP_ERR_INVALID, std::string("write failed: ") + path);
+  return OCP_OK;
+}
+
+int ocp_cdf_load(ocp_ctx c, const char* path) {
+  if (check_ctx(c) || !path) return OCP_ERR_INVALID;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(c, OCP_ERR_INVALID, std::string("cannot open ") + path);
+  char magic[8];
+  uint32_t hdr[6];
+  std::vector<float> p(kCdfParams);
+  bool ok = std::fread(magic, 1, 8, f) == 8 && std::fread(hdr, 4, 6, f) == 6;
+  ok = ok && std::memcmp(magic, kCdfMagic, 8) == 0 && hdr[0] == 1u &&
+       hdr[1] == static_cast<uint32_t>(kCdfParams) && hdr[4] == static_cast<uint32_t>(kCdfHidden) &&
+       hdr[5] == static_cast<uint32_t>(kCdfLeaves) && hdr[3] >= hdr[2];
+  ok = ok && std::fread(p.data(), 4, p.size(), f) == p.size();
+  char extra;
+  ok = ok && std::fread(&extra, 1, 1, f) == 0;
+  std::fclose(f);
+  if (!ok) return fail(c, OCP_ERR_INVALID, std::string("not a CDF model file: ") + path);
+  return ocp_cdf_set(c, p.data(), hdr[2], hdr[3]);
+}
+
+int ocp_cdf_grad(ocp_ctx c, const float* xs, const float* ts, int32_t n, float* grad) {
+  if (check_ctx(c) || !grad) return OCP_ERR_INVALID;
+  if (n < 1 || !xs || !ts) return fail(c, OCP_ERR_INVALID, "need n >= 1 samples");
+  LearnState& L = c->learn;
+  if (!L.model) return fail(c, OCP_ERR_STATE, "no CDF model");
+  float* d_grad = nullptr;
+  if (int r = dev_alloc(c, &d_grad, kCdfRootParams)) return r;
+  launch_fill_u32(reinterpret_cast<uint32_t*>(d_grad), 0u, kCdfRootParams, c->stream);
+  launch_lgrad(xs, ts, n, L.model, d_grad, c->stream);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(grad, d_grad, sizeof(float) * kCdfRootParams, hipMemcpyDeviceToHost,
+                       c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  dev_free(d_grad);
+  if (e != hipSuccess) return fail(c, OCP_ERR_HIP, hipGetErrorString(e));
+  return OCP_OK;
+}
+
+int ocp_cdf_buckets(ocp_ctx c, const uint32_t* keys, uint32_t n, int mode, uint32_t* q) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (n > 0 && (!keys || !q)) return fail(c, OCP_ERR_INVALID, "null pointer");
+  if (mode != 0 && mode != 1) return fail(c, OCP_ERR_INVALID, "mode must be 0 or 1");
+  if (n > kMaxPoints) return fail(c, OCP_ERR_CAPACITY, "n > 2^30-1");
+  LearnState& L = c->learn;
+  if (!L.compiled) return fail(c, OCP_ERR_STATE, "no CDF model: train, set, reset or load one");
+  launch_lbucket(keys, static_cast<int32_t>(n), L.model, L.meta, L.kleaf, L.fb, mode, q, c->stream);
+  OCP_CHECK_LAUNCH(c);
+  return OCP_OK;
+}
+
+int ocp_cdf_boundaries(ocp_ctx c, uint32_t* fb) {
+  if (check_ctx(c) || !fb) return OCP_ERR_INVALID;
+  LearnState& L = c->learn;
+  if (!L.compiled) return fail(c, OCP_ERR_STATE, "no CDF model: train, set, reset or load one");
+  OCP_HIP(c, hipMemcpyAsync(fb, L.fb, sizeof(uint32_t) * kLearnBuckets, hipMemcpyDeviceToHost,
+                            c->stream));
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  return OCP_OK;
+}
+
+int ocp_sort_learned(ocp_ctx c, uint32_t* keys, uint32_t* alt, uint32_t n,
+                     ocp_learned_stats* st) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  ocp_learned_stats local{};
+  local.n = n;
+  if (n > 0 && (!keys || !alt)) return fail(c, OCP_ERR_INVALID, "null pointer");
+  if (n > kMaxPoints) return fail(c, OCP_ERR_CAPACITY, "n > 2^30-1");
+  LearnState& L = c->learn;
+  if (!L.compiled) return fail(c, OCP_ERR_STATE, "no CDF model: train, set, reset or load one");
+  if (n <= 1) {
+    if (st) *st = local;
+    return OCP_OK;
+  }
+  if (int r = learn_tables_alloc(c, n)) return r;
+  LearnTables& tb = L.tb;
+  const int32_t ni = static_cast<int32_t>(n);
+  const int stile = sort_tile(c->tune);
+  const int64_t fb_tiles = cdiv(n, stile) + tb.big_cap;
+  if (int r = ensure_scratch(c, std::max<int64_t>(learn_pass_b_max_tiles(ni), fb_tiles) * 256))
+    return r;
+  launch_begin(c->ctrl, nullptr, 0, c->stream);
+  int32_t rows = 0;
+  launch_lhist(keys, n, tb, &rows, c->stream);
+  launch_lprefix(tb, rows, stile, c->stream);
+  OCP_HIP(c, hipMemcpyAsync(L.h_info, tb.info, sizeof(int32_t) * 8, hipMemcpyDeviceToHost,
+                            c->stream));
+  OCP_HIP(c, hipEventRecord(L.ev_info, c->stream));
+  launch_lpass_a(keys, alt, ni, tb, c->s_flags, c->ctrl, c->stream);
+  launch_lpass_b(alt, keys, ni, tb, c->s_flags, c->ctrl, c->stream);
+  launch_lsort(keys, tb, c->stream);
+  OCP_CHECK_LAUNCH(c);
+  // The bucket statistics were copied right after the tables were built;
+  // waiting for them does not wait for the scatter passes.
+  OCP_HIP(c, hipEventSynchronize(L.ev_info));
+  const int32_t n_big = L.h_info[kInfoBig];
+  const int32_t big_tiles = L.h_info[kInfoBigTiles];
+  local.n_buckets_nonempty = L.h_info[kInfoNonEmpty];
+  local.max_bucket = L.h_info[kInfoMaxBucket];
+  local.n_big = n_big;
+  local.keys_big = static_cast<uint32_t>(L.h_info[kInfoBigKeys]);
+  if (n_big > tb.big_cap) return fail(c, OCP_ERR_STATE, "oversized-bucket table overflow");
+  if (n_big > 0) {
+    // Oversized buckets: segmented OneSweep, in place (4 passes end in keys).
+    launch_fill_u32(L.big_hist, 0u, int64_t(n_big) * 1024, c->stream);
+    SegView sv{};
+    sv.n_seg = n_big;
+    sv.start = tb.big_start;
+    sv.end = tb.big_end;
+    sv.n1 = ni;
+    launch_histogram_tiles(keys, sv, tb.big_tiles, big_tiles, stile, L.big_hist, c->tune,
+                           c->stream);
+    uint32_t* src = keys;
+    uint32_t* dst = alt;
+    for (int pass = 0; pass < 4; ++pass) {
+      launch_onesweep_pass(src, dst, sv, tb.big_tiles, big_tiles, L.big_hist, c->s_flags, c->ctrl,
+                           pass, c->tune, c->stream);
+      std::swap(src, dst);
+    }
+  }
+  launch_lverify(keys, tb, ni, c->ctrl, c->stream);
+  OCP_CHECK_LAUNCH(c);
+  uint32_t cw[kCtrlWords];
+  OCP_HIP(c, hipMemcpyAsync(cw, c->ctrl, sizeof(cw), hipMemcpyDeviceToHost, c->stream));
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  if (cw[kCtrlTimeout]) {
+    OCP_HIP(c, hipMemsetAsync(c->ctrl + kCtrlTimeout, 0, sizeof(uint32_t), c->stream));
+    return fail(c, OCP_ERR_STATE, "a lookback wait hit its spin limit; the result is invalid");
+  }
+  local.misordered = static_cast<int32_t>(cw[kCtrlLearnBad]);
+  if (local.misordered > 0) {
+    if (int r = ocp_k_SortKeys(c, keys, alt, n)) return r;
+    OCP_HIP(c, hipStreamSynchronize(c->stream));
+    local.full_resort = 1;
+  }
+  if (st) *st = local;
   return OCP_OK;
 }
 

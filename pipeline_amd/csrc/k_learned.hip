@@ -1,0 +1,811 @@
+// Learned-CDF bucket sort of 32-bit keys, and the kernels that train and
+// compile its model.
+//
+// Model (FP32, layout in learned.hpp): a two-level piecewise-linear CDF.
+//   root  r(x) = b2 + sum_j w2_j * relu(w1_j * x + b1_j), 16 hidden units,
+//         x = (key - kmin) / (kmax - kmin + 1); after every SGD step
+//         w1_j * w2_j >= 0 is restored, so each term, and r, is
+//         non-decreasing in x (float rounding keeps that: every operation
+//         is monotone in its operands);
+//   leaf  l(key) = clamp(floor(r * 4096), 0, 4095); leaf l starts at key
+//         K_l = min{key : l(key) >= l} and maps
+//         y = clamp(a_l + b_l * (key - K_l), lo_l, lo_{l+1}), b_l >= 0,
+//         lo_l = the sample CDF at K_l;
+//   bucket q(key) = clamp(floor(y * 16384), 0, 16383).
+// q is non-decreasing in the key, so it is fully described by its 16384
+// bucket boundaries fb[f] = min{key : q(key) >= f}: q(key) = #{f >= 1 :
+// fb[f] <= key}. The compile step finds every fb[f] by binary search over
+// the 32-bit key space with the model itself; the sort evaluates q only
+// through fb (a binary search in LDS), never the network.
+//
+// Sort (keys only, so stability is irrelevant and any correct per-bucket
+// sort gives the bytes of np.sort):
+//   1. histogram of q (persistent workgroups, fb and counters in LDS);
+//   2. one workgroup turns it into bucket starts, the two scatter passes'
+//      digit tables and the list of oversized buckets;
+//   3. OneSweep scatter by the coarse digit q >> 6, then per coarse segment
+//      by the fine digit q & 63 (onesweep_impl.hpp, stable decoupled
+//      lookback), which leaves every bucket contiguous at its start;
+//   4. one workgroup per bucket sorts it in LDS (LSD radix over the bits its
+//      keys span above the bucket minimum), in three size classes;
+//   5. buckets above 16384 keys are sorted by the segmented OneSweep
+//      (host side); constant buckets need nothing;
+//   6. every bucket boundary is checked (a misorder makes the host re-sort
+//      the whole array with OneSweep).
+#include "learned.hpp"
+#include "onesweep_impl.hpp"
+
+namespace ocp {
+namespace {
+
+// ------------------------------------------------------------ model eval
+struct CdfDev {
+  const float* p;
+  const uint32_t* kleaf;
+  uint32_t kmin;
+  float inv_span;
+};
+
+__device__ __forceinline__ CdfDev cdf_view(const float* p, const uint32_t* meta,
+                                           const uint32_t* kleaf) {
+  CdfDev v;
+  v.p = p;
+  v.kleaf = kleaf;
+  v.kmin = meta[0];
+  v.inv_span = 1.0f / (static_cast<float>(meta[1] - meta[0]) + 1.0f);
+  return v;
+}
+
+__device__ __forceinline__ float cdf_x(const CdfDev& v, uint32_t key) {
+  const uint32_t d = key > v.kmin ? key - v.kmin : 0u;
+  return static_cast<float>(d) * v.inv_span;
+}
+
+__device__ __forceinline__ float root_eval(const float* __restrict__ m, float x) {
+  float r = m[kCdfB2];
+#pragma unroll
+  for (int j = 0; j < kCdfHidden; ++j) {
+    const float h = fmaxf(m[kCdfW1 + j] * x + m[kCdfB1 + j], 0.0f);
+    r = r + m[kCdfW2 + j] * h;
+  }
+  return r;
+}
+
+// The maximal key always takes the last leaf and bucket, so every boundary
+// exists inside the 32-bit key space.
+__device__ __forceinline__ int32_t leaf_of(const CdfDev& v, uint32_t key) {
+  if (key == 0xffffffffu) return kCdfLeaves - 1;
+  const float s = floorf(root_eval(v.p, cdf_x(v, key)) * static_cast<float>(kCdfLeaves));
+  if (!(s > 0.0f)) return 0;  // also NaN
+  return s >= static_cast<float>(kCdfLeaves) ? kCdfLeaves - 1 : static_cast<int32_t>(s);
+}
+
+__device__ __forceinline__ uint32_t q_model(const CdfDev& v, uint32_t key) {
+  if (key == 0xffffffffu) return kLearnBuckets - 1;
+  const int32_t l = leaf_of(v, key);
+  const float d = static_cast<float>(key - v.kleaf[l]);
+  const float lo = v.p[kCdfLeafLo + l];
+  const float hi = v.p[kCdfLeafLo + l + 1];
+  float y = v.p[kCdfLeafA + l] + v.p[kCdfLeafB + l] * d;
+  y = fminf(fmaxf(y, lo), hi);
+  const float s = floorf(y * static_cast<float>(kLearnBuckets));
+  if (!(s > 0.0f)) return 0u;
+  return s >= static_cast<float>(kLearnBuckets) ? kLearnBuckets - 1u : static_cast<uint32_t>(s);
+}
+
+// q through the boundaries, by branch-free binary search: `cb` holds
+// fb[64 c] for c < 256, `fbs` the 64 boundaries of one coarse segment.
+__device__ __forceinline__ uint32_t coarse_of(const uint32_t* cb, uint32_t key) {
+  uint32_t c = 0;
+#pragma unroll
+  for (uint32_t step = 128; step > 0; step >>= 1) c += cb[c + step] <= key ? step : 0u;
+  return c;
+}
+__device__ __forceinline__ uint32_t fine_of(const uint32_t* fbs, uint32_t key) {
+  uint32_t f = 0;
+#pragma unroll
+  for (uint32_t step = 32; step > 0; step >>= 1) f += fbs[f + step] <= key ? step : 0u;
+  return f;
+}
+
+// Digit functors of the two scatter passes; their tables live in LDS.
+__shared__ uint32_t s_cb[kLearnCoarse];
+__shared__ uint32_t s_fbs[kLearnFine];
+
+struct CoarseDigit {
+  const uint32_t* fb;
+  __device__ __forceinline__ void setup(int32_t, int tid, int threads) const {
+    for (int i = tid; i < kLearnCoarse; i += threads) s_cb[i] = fb[i * kLearnFine];
+  }
+  __device__ __forceinline__ uint32_t operator()(uint32_t key) const { return coarse_of(s_cb, key); }
+};
+struct FineDigit {
+  const uint32_t* fb;
+  __device__ __forceinline__ void setup(int32_t seg, int tid, int threads) const {
+    for (int i = tid; i < kLearnFine; i += threads) s_fbs[i] = fb[seg * kLearnFine + i];
+  }
+  __device__ __forceinline__ uint32_t operator()(uint32_t key) const { return fine_of(s_fbs, key); }
+};
+
+// ------------------------------------------------------------ histogram
+constexpr int kHistThreads = 1024;
+
+__global__ __launch_bounds__(kHistThreads) void lhist_kernel(const uint32_t* __restrict__ keys,
+                                                             int64_t n,
+                                                             const uint32_t* __restrict__ fb,
+                                                             uint32_t* __restrict__ rows) {
+  __shared__ uint32_t s_fb[kLearnBuckets];
+  __shared__ uint32_t s_cnt[kLearnBuckets];
+  __shared__ uint32_t s_c[kLearnCoarse];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < kLearnBuckets; i += kHistThreads) {
+    s_fb[i] = fb[i];
+    s_cnt[i] = 0;
+  }
+  for (int i = tid; i < kLearnCoarse; i += kHistThreads) s_c[i] = fb[i * kLearnFine];
+  __syncthreads();
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kHistThreads;
+  int64_t i = static_cast<int64_t>(blockIdx.x) * kHistThreads + tid;
+  constexpr int U = 4;
+  for (; i + (U - 1) * stride < n; i += U * stride) {
+    uint32_t k[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) k[u] = keys[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t c = coarse_of(s_c, k[u]);
+      const uint32_t f = fine_of(s_fb + c * kLearnFine, k[u]);
+      atomicAdd(&s_cnt[c * kLearnFine + f], 1u);
+    }
+  }
+  for (; i < n; i += stride) {
+    const uint32_t key = keys[i];
+    const uint32_t c = coarse_of(s_c, key);
+    const uint32_t f = fine_of(s_fb + c * kLearnFine, key);
+    atomicAdd(&s_cnt[c * kLearnFine + f], 1u);
+  }
+  __syncthreads();
+  uint32_t* row = rows + static_cast<int64_t>(blockIdx.x) * kLearnBuckets;
+  for (int q = tid; q < kLearnBuckets; q += kHistThreads) row[q] = s_cnt[q];
+}
+
+__global__ __launch_bounds__(256) void lreduce_kernel(const uint32_t* __restrict__ rows,
+                                                      int32_t n_rows,
+                                                      uint32_t* __restrict__ hist) {
+  const int32_t q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= kLearnBuckets) return;
+  uint32_t s = 0;
+  for (int32_t r = 0; r < n_rows; ++r) s += rows[static_cast<int64_t>(r) * kLearnBuckets + q];
+  hist[q] = s;
+}
+
+// ------------------------------------------------------------ tables
+// One workgroup of 1024 threads, 16 buckets per thread.
+__global__ __launch_bounds__(1024) void lprefix_kernel(LearnTables tb, int32_t fallback_tile) {
+  __shared__ uint32_t tmp[16];
+  __shared__ int32_t s_bs[kLearnBuckets + 1];
+  __shared__ int32_t s_max;
+  constexpr int PER = kLearnBuckets / 1024;
+  const int tid = threadIdx.x;
+  if (tid == 0) s_max = 0;
+  uint32_t v[PER];
+  uint32_t sum = 0, nonempty = 0, nbig = 0, big_keys = 0, big_tiles = 0;
+  int32_t mx = 0;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    v[q] = tb.hist[tid * PER + q];
+    sum += v[q];
+    nonempty += v[q] ? 1u : 0u;
+    mx = max(mx, static_cast<int32_t>(v[q]));
+    if (v[q] > static_cast<uint32_t>(kLearnBigCap)) {
+      ++nbig;
+      big_keys += v[q];
+      big_tiles += (v[q] + fallback_tile - 1) / fallback_tile;
+    }
+  }
+  uint32_t total;
+  uint32_t run = block_excl_scan<1024>(sum, tmp, &total);
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    s_bs[tid * PER + q] = static_cast<int32_t>(run);
+    tb.bstart[tid * PER + q] = static_cast<int32_t>(run);
+    run += v[q];
+  }
+  if (tid == 0) {
+    s_bs[kLearnBuckets] = static_cast<int32_t>(total);
+    tb.bstart[kLearnBuckets] = static_cast<int32_t>(total);
+  }
+  atomicMax(&s_max, mx);
+  // oversized buckets, in bucket order
+  uint32_t n_big_all, n_tiles_all, keys_all, ne_all;
+  uint32_t bi = block_excl_scan<1024>(nbig, tmp, &n_big_all);
+  uint32_t bt = block_excl_scan<1024>(big_tiles, tmp, &n_tiles_all);
+  (void)block_excl_scan<1024>(big_keys, tmp, &keys_all);
+  (void)block_excl_scan<1024>(nonempty, tmp, &ne_all);
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    if (v[q] > static_cast<uint32_t>(kLearnBigCap) && static_cast<int32_t>(bi) < tb.big_cap) {
+      const int32_t b = tid * PER + q;
+      tb.big_start[bi] = s_bs[b];
+      tb.big_end[bi] = s_bs[b + 1];
+      tb.big_tiles[bi] = static_cast<int32_t>(bt);
+      bt += (v[q] + fallback_tile - 1) / fallback_tile;
+      ++bi;
+    }
+  }
+  if (tid == 0) tb.big_tiles[min(static_cast<int32_t>(n_big_all), tb.big_cap)] =
+      static_cast<int32_t>(n_tiles_all);
+  // coarse segments: counts, starts, tile map; fine counts per segment
+  for (int i = tid; i < kLearnBuckets; i += 1024)
+    tb.hist_b[(i / kLearnFine) * 1024 + (i % kLearnFine)] = tb.hist[i];
+  uint32_t cnt = 0;
+  if (tid < kLearnCoarse) {
+    cnt = static_cast<uint32_t>(s_bs[(tid + 1) * kLearnFine] - s_bs[tid * kLearnFine]);
+    tb.hist_a[tid] = cnt;
+    tb.seg_start[tid] = s_bs[tid * kLearnFine];
+  }
+  const uint32_t tiles = (cnt + kLearnPassTile - 1) / kLearnPassTile;
+  uint32_t ttot;
+  const uint32_t tex = block_excl_scan<1024>(tiles, tmp, &ttot);
+  if (tid < kLearnCoarse) tb.tile_start[tid] = static_cast<int32_t>(tex);
+  if (tid == 0) {
+    tb.seg_start[kLearnCoarse] = static_cast<int32_t>(total);
+    tb.tile_start[kLearnCoarse] = static_cast<int32_t>(ttot);
+    *tb.tiles_b = static_cast<int32_t>(ttot);
+    tb.info[kInfoBig] = static_cast<int32_t>(n_big_all);
+    tb.info[kInfoBigTiles] = static_cast<int32_t>(n_tiles_all);
+    tb.info[kInfoBigKeys] = static_cast<int32_t>(keys_all);
+    tb.info[kInfoMaxBucket] = s_max;
+    tb.info[kInfoNonEmpty] = static_cast<int32_t>(ne_all);
+  }
+}
+
+// ------------------------------------------------------------ bucket sort
+// One workgroup per bucket whose size is in (lo_cap, hi_cap]: keys are
+// loaded wave-contiguously, taken relative to the bucket minimum, and
+// sorted by stable LSD passes of 8 bits over the bits they span, ranked
+// per wave by ballot multisplit and scattered through LDS.
+template <int WAVES, int KPT>
+__global__ __launch_bounds__(WAVES * 64) void lsort_kernel(uint32_t* __restrict__ keys,
+                                                           const int32_t* __restrict__ bstart,
+                                                           int32_t lo_cap) {
+  constexpr int THREADS = WAVES * 64;
+  constexpr int CAP = THREADS * KPT;
+  __shared__ uint32_t s_keys[CAP];
+  __shared__ uint32_t s_whist[WAVES * 256];
+  __shared__ uint32_t s_dstart[256];
+  __shared__ uint32_t s_tmp[WAVES];
+  __shared__ uint32_t s_mm[2];
+  const int32_t b = blockIdx.x;
+  const int32_t lo = bstart[b];
+  const int32_t n = bstart[b + 1] - lo;
+  if (n <= lo_cap || n > CAP) return;
+  const int tid = threadIdx.x;
+  const uint32_t w = wave_id();
+  const uint32_t lane = lane_id();
+  uint32_t* g = keys + lo;
+  if (tid == 0) {
+    s_mm[0] = 0xffffffffu;
+    s_mm[1] = 0u;
+  }
+  uint32_t kv[KPT];
+  uint32_t mn = 0xffffffffu, mx = 0u;
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
+    kv[k] = idx < n ? g[idx] : 0u;
+    if (idx < n) {
+      mn = min(mn, kv[k]);
+      mx = max(mx, kv[k]);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    mn = min(mn, static_cast<uint32_t>(__shfl_xor(static_cast<int>(mn), o, 64)));
+    mx = max(mx, static_cast<uint32_t>(__shfl_xor(static_cast<int>(mx), o, 64)));
+  }
+  __syncthreads();
+  if (lane == 0) {
+    atomicMin(&s_mm[0], mn);
+    atomicMax(&s_mm[1], mx);
+  }
+  __syncthreads();
+  mn = s_mm[0];
+  mx = s_mm[1];
+  if (mn == mx) return;  // constant bucket: already sorted
+  const int bits = 32 - __clz(static_cast<int>(mx - mn));
+  const int passes = (bits + 7) / 8;
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
+    kv[k] = idx < n ? kv[k] - mn : 0xffffffffu;  // padding ranks last
+  }
+  const uint64_t lt = lanemask_lt();
+  uint32_t* wh = s_whist + w * 256;
+  for (int pass = 0; pass < passes; ++pass) {
+    const uint32_t shift = 8u * static_cast<uint32_t>(pass);
+    for (int i = tid; i < WAVES * 256; i += THREADS) s_whist[i] = 0;
+    __syncthreads();
+    uint32_t off[KPT];
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      const uint32_t d = (kv[k] >> shift) & 255u;
+      uint64_t msk = ~0ull;
+#pragma unroll
+      for (int bb = 0; bb < 8; ++bb) {
+        const bool bit = (d >> bb) & 1u;
+        const uint64_t bal = __ballot(bit);
+        msk &= bit ? bal : ~bal;
+      }
+      const uint32_t rank = __popcll(msk & lt);
+      const uint32_t old = wh[d];
+      if (rank == 0) wh[d] = old + __popcll(msk);
+      off[k] = old + rank;
+    }
+    __syncthreads();
+    uint32_t tot = 0;
+    if (tid < 256) {
+      uint32_t run = 0;
+#pragma unroll
+      for (int ww = 0; ww < WAVES; ++ww) {
+        const uint32_t cc = s_whist[ww * 256 + tid];
+        s_whist[ww * 256 + tid] = run;
+        run += cc;
+      }
+      tot = run;
+    }
+    uint32_t all;
+    const uint32_t ds = block_excl_scan<THREADS>(tot, s_tmp, &all);
+    if (tid < 256) s_dstart[tid] = ds;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      const uint32_t d = (kv[k] >> shift) & 255u;
+      s_keys[s_dstart[d] + wh[d] + off[k]] = kv[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KPT; ++k)
+      kv[k] = s_keys[static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane)];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
+    if (idx < n) g[idx] = kv[k] + mn;
+  }
+}
+
+// The last key of each non-empty bucket must not exceed the next key.
+__global__ __launch_bounds__(256) void lverify_kernel(const uint32_t* __restrict__ keys,
+                                                      const int32_t* __restrict__ bstart,
+                                                      int32_t n, uint32_t* __restrict__ ctrl) {
+  const int32_t b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= kLearnBuckets) return;
+  const int32_t s = bstart[b], e = bstart[b + 1];
+  if (e > s && e < n && keys[e - 1] > keys[e]) atomicAdd(&ctrl[kCtrlLearnBad], 1u);
+}
+
+// ------------------------------------------------------------ training
+__global__ __launch_bounds__(256) void lsample_kernel(const uint32_t* __restrict__ keys,
+                                                      int32_t stride,
+                                                      uint32_t* __restrict__ ring,
+                                                      int64_t ring_cap, int64_t ring_pos,
+                                                      int64_t count) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < count) ring[(ring_pos + i) % ring_cap] = keys[i * stride];
+}
+
+__global__ void lmeta_kernel(const uint32_t* __restrict__ sorted, int32_t ns,
+                             uint32_t* __restrict__ meta) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    meta[0] = sorted[0];
+    meta[1] = sorted[ns - 1];
+  }
+}
+
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// Mini-batch SGD with momentum on the mean of 0.5 * (r(x) - t)^2, sample i
+// of the sorted samples having target t = (i + 0.5) / ns. One workgroup of
+// 1024 threads, one sample per thread per step; the gradient is summed over
+// the workgroup in a fixed order, so training is deterministic.
+__global__ __launch_bounds__(1024) void ltrain_root_kernel(const uint32_t* __restrict__ sorted,
+                                                           int32_t ns,
+                                                           const uint32_t* __restrict__ meta,
+                                                           int32_t steps, float lr,
+                                                           float momentum, uint32_t seed,
+                                                           float* __restrict__ model,
+                                                           float* __restrict__ loss_out) {
+  constexpr int P = kCdfRootParams;
+  __shared__ float s_m[P];
+  __shared__ float s_v[P];
+  __shared__ float s_part[16][P + 1];
+  const int tid = threadIdx.x;
+  const uint32_t w = wave_id();
+  const uint32_t lane = lane_id();
+  const CdfDev cv = cdf_view(model, meta, nullptr);
+  if (tid < P) {
+    s_m[tid] = model[tid];
+    s_v[tid] = 0.0f;
+  }
+  __syncthreads();
+  const float inv_ns = 1.0f / static_cast<float>(ns);
+  float last_loss = 0.0f;
+  for (int32_t step = 0; step < steps; ++step) {
+    const uint32_t idx =
+        hash32(seed ^ hash32(static_cast<uint32_t>(step) * 1024u + static_cast<uint32_t>(tid))) %
+        static_cast<uint32_t>(ns);
+    const float x = cdf_x(cv, sorted[idx]);
+    const float t = (static_cast<float>(idx) + 0.5f) * inv_ns;
+    float h[kCdfHidden];
+    float r = s_m[kCdfB2];
+#pragma unroll
+    for (int j = 0; j < kCdfHidden; ++j) {
+      h[j] = fmaxf(s_m[kCdfW1 + j] * x + s_m[kCdfB1 + j], 0.0f);
+      r = r + s_m[kCdfW2 + j] * h[j];
+    }
+    const float err = r - t;
+    last_loss = 0.5f * err * err;
+    float gr[P];
+#pragma unroll
+    for (int j = 0; j < kCdfHidden; ++j) {
+      const float dh = h[j] > 0.0f ? err * s_m[kCdfW2 + j] : 0.0f;
+      gr[kCdfW1 + j] = dh * x;
+      gr[kCdfB1 + j] = dh;
+      gr[kCdfW2 + j] = err * h[j];
+    }
+    gr[kCdfB2] = err;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      float g = gr[p];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) g += __shfl_xor(g, o, 64);
+      if (lane == 0) s_part[w][p] = g;
+    }
+    __syncthreads();
+    if (tid < P) {
+      float g = 0.0f;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) g += s_part[q][tid];
+      g *= (1.0f / 1024.0f);
+      s_v[tid] = momentum * s_v[tid] + g;
+      s_m[tid] = s_m[tid] - lr * s_v[tid];
+    }
+    __syncthreads();
+    if (tid < kCdfHidden && s_m[kCdfW1 + tid] * s_m[kCdfW2 + tid] < 0.0f) {
+      s_m[kCdfW2 + tid] = 0.0f;
+      s_v[kCdfW2 + tid] = 0.0f;
+    }
+    __syncthreads();
+  }
+  if (tid < P) model[tid] = s_m[tid];
+  float l = last_loss;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o, 64);
+  if (lane == 0) s_part[w][0] = l;
+  __syncthreads();
+  if (tid == 0) {
+    float s = 0.0f;
+    for (int q = 0; q < 16; ++q) s += s_part[q][0];
+    *loss_out = s * (1.0f / 1024.0f);
+  }
+}
+
+// K_l = min{key : l(key) >= l} by binary search over the key space
+// (l(key) is non-decreasing and l(0xffffffff) = 4095); K_4096 = 0xffffffff.
+__global__ __launch_bounds__(256) void lleaf_starts_kernel(const float* __restrict__ model,
+                                                           const uint32_t* __restrict__ meta,
+                                                           uint32_t* __restrict__ kleaf) {
+  const int32_t l = blockIdx.x * 256 + threadIdx.x;
+  if (l > kCdfLeaves) return;
+  if (l == 0 || l == kCdfLeaves) {
+    kleaf[l] = l == 0 ? 0u : 0xffffffffu;
+    return;
+  }
+  const CdfDev v = cdf_view(model, meta, nullptr);
+  uint32_t lo = 0, hi = 0xffffffffu;
+  while (lo < hi) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    if (leaf_of(v, mid) >= l) hi = mid;
+    else lo = mid + 1;
+  }
+  kleaf[l] = lo;
+}
+
+__device__ __forceinline__ int32_t lower_bound_u32(const uint32_t* a, int32_t n, uint32_t key) {
+  int32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Leaf l: lo_l = the sample CDF at K_l; (a_l, b_l) = least squares of
+// t = a + b * (key - K_l) over the samples in [K_l, K_{l+1}), b >= 0.
+__global__ __launch_bounds__(256) void lfit_leaves_kernel(const uint32_t* __restrict__ sorted,
+                                                          int32_t ns,
+                                                          const uint32_t* __restrict__ kleaf,
+                                                          float* __restrict__ model) {
+  const int32_t l = blockIdx.x * 256 + threadIdx.x;
+  if (l == 0) model[kCdfLeafLo + kCdfLeaves] = 1.0f;
+  if (l >= kCdfLeaves) return;
+  const uint32_t k0 = kleaf[l];
+  const int32_t s = l == 0 ? 0 : lower_bound_u32(sorted, ns, k0);
+  const int32_t e = l == kCdfLeaves - 1 ? ns : lower_bound_u32(sorted, ns, kleaf[l + 1]);
+  const double inv = 1.0 / ns;
+  double sd = 0, st = 0, sdd = 0, sdt = 0;
+  for (int32_t i = s; i < e; ++i) {
+    const double d = static_cast<double>(sorted[i] - k0);
+    const double t = (i + 0.5) * inv;
+    sd += d;
+    st += t;
+    sdd += d * d;
+    sdt += d * t;
+  }
+  const double cnt = e - s;
+  double a = s * inv, b = 0.0;
+  if (cnt >= 1) a = st / cnt;
+  if (cnt >= 2) {
+    const double den = cnt * sdd - sd * sd;
+    if (den > 0.0) {
+      b = (cnt * sdt - sd * st) / den;
+      if (b < 0.0) b = 0.0;
+      a = (st - b * sd) / cnt;
+    }
+  }
+  model[kCdfLeafLo + l] = static_cast<float>(s * inv);
+  model[kCdfLeafA + l] = static_cast<float>(a);
+  model[kCdfLeafB + l] = static_cast<float>(b);
+}
+
+// Linear-CDF leaves for the current root: y runs linearly from l / 4096 at
+// K_l to (l + 1) / 4096 at K_{l+1} (the last leaf ends at kmax + 1).
+__global__ __launch_bounds__(256) void llinear_leaves_kernel(const uint32_t* __restrict__ kleaf,
+                                                             const uint32_t* __restrict__ meta,
+                                                             float* __restrict__ model) {
+  const int32_t l = blockIdx.x * 256 + threadIdx.x;
+  if (l > kCdfLeaves) return;
+  model[kCdfLeafLo + l] = static_cast<float>(l) / static_cast<float>(kCdfLeaves);
+  if (l == kCdfLeaves) return;
+  const uint64_t k0 = kleaf[l];
+  const uint64_t k1 = l == kCdfLeaves - 1 ? static_cast<uint64_t>(meta[1]) + 1u : kleaf[l + 1];
+  const double span = k1 > k0 ? static_cast<double>(k1 - k0) : 1.0;
+  model[kCdfLeafA + l] = static_cast<float>(l) / static_cast<float>(kCdfLeaves);
+  model[kCdfLeafB + l] = static_cast<float>(1.0 / (kCdfLeaves * span));
+}
+
+// Identity root r(x) = x, the start of training: units 0..7 are
+// relu(x - j/8) (convex kinks), units 8..15 relu((j - 7.5)/8 - x) (concave
+// kinks once their output weight turns negative); only unit 0 has a
+// non-zero output weight. Units of both directions are needed: with
+// w1 * w2 >= 0, units with w1 > 0 can only add convex kinks.
+__global__ void llinear_root_kernel(float* __restrict__ model) {
+  const int j = threadIdx.x;
+  if (j < kCdfHidden) {
+    const bool up = j < kCdfHidden / 2;
+    model[kCdfW1 + j] = up ? 1.0f : -1.0f;
+    model[kCdfB1 + j] = up ? -static_cast<float>(j) / 8.0f : (static_cast<float>(j) - 7.5f) / 8.0f;
+    model[kCdfW2 + j] = j == 0 ? 1.0f : 0.0f;
+  }
+  if (j == 0) model[kCdfB2] = 0.0f;
+}
+
+// fb[f] = min{key : q(key) >= f}, then a running maximum so that the
+// boundaries are non-decreasing whatever the parameters are.
+__global__ __launch_bounds__(256) void lboundaries_kernel(const float* __restrict__ model,
+                                                          const uint32_t* __restrict__ meta,
+                                                          const uint32_t* __restrict__ kleaf,
+                                                          uint32_t* __restrict__ fb) {
+  const int32_t f = blockIdx.x * 256 + threadIdx.x;
+  if (f >= kLearnBuckets) return;
+  if (f == 0) {
+    fb[0] = 0u;
+    return;
+  }
+  const CdfDev v = cdf_view(model, meta, kleaf);
+  uint32_t lo = 0, hi = 0xffffffffu;
+  while (lo < hi) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    if (q_model(v, mid) >= static_cast<uint32_t>(f)) hi = mid;
+    else lo = mid + 1;
+  }
+  fb[f] = lo;
+}
+
+__global__ __launch_bounds__(1024) void lmonotone_kernel(uint32_t* __restrict__ fb) {
+  __shared__ uint32_t s_m[1024];
+  constexpr int PER = kLearnBuckets / 1024;
+  const int tid = threadIdx.x;
+  uint32_t v[PER];
+  uint32_t m = 0;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    v[q] = fb[tid * PER + q];
+    m = max(m, v[q]);
+  }
+  s_m[tid] = m;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t run = 0;
+    for (int i = 0; i < 1024; ++i) {
+      const uint32_t x = s_m[i];
+      s_m[i] = run;
+      run = max(run, x);
+    }
+  }
+  __syncthreads();
+  uint32_t run = s_m[tid];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    run = max(run, v[q]);
+    fb[tid * PER + q] = run;
+  }
+}
+
+// ------------------------------------------------------------ diagnostics
+// Mean gradient of 0.5 * (r(x) - t)^2 over n pairs, accumulated into grad
+// (zeroed by the caller).
+__global__ __launch_bounds__(256) void lgrad_kernel(const float* __restrict__ xs,
+                                                    const float* __restrict__ ts, int32_t n,
+                                                    const float* __restrict__ model,
+                                                    float* __restrict__ grad) {
+  const int32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float x = xs[i], t = ts[i];
+  float h[kCdfHidden];
+  float r = model[kCdfB2];
+#pragma unroll
+  for (int j = 0; j < kCdfHidden; ++j) {
+    h[j] = fmaxf(model[kCdfW1 + j] * x + model[kCdfB1 + j], 0.0f);
+    r = r + model[kCdfW2 + j] * h[j];
+  }
+  const float err = r - t;
+  const float sc = 1.0f / static_cast<float>(n);
+#pragma unroll
+  for (int j = 0; j < kCdfHidden; ++j) {
+    const float dh = h[j] > 0.0f ? err * model[kCdfW2 + j] : 0.0f;
+    atomicAdd(&grad[kCdfW1 + j], dh * x * sc);
+    atomicAdd(&grad[kCdfB1 + j], dh * sc);
+    atomicAdd(&grad[kCdfW2 + j], err * h[j] * sc);
+  }
+  atomicAdd(&grad[kCdfB2], err * sc);
+}
+
+__global__ __launch_bounds__(256) void lbucket_kernel(const uint32_t* __restrict__ keys,
+                                                      int32_t n, const float* __restrict__ model,
+                                                      const uint32_t* __restrict__ meta,
+                                                      const uint32_t* __restrict__ kleaf,
+                                                      const uint32_t* __restrict__ fb, int mode,
+                                                      uint32_t* __restrict__ q) {
+  const int32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t key = keys[i];
+  if (mode == 0) {
+    q[i] = q_model(cdf_view(model, meta, kleaf), key);
+  } else {
+    uint32_t f = 0;
+    for (uint32_t step = kLearnBuckets / 2; step > 0; step >>= 1)
+      f += fb[f + step] <= key ? step : 0u;
+    q[i] = f;
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ launchers
+void launch_lhist(const uint32_t* keys, int64_t n, const LearnTables& tb, int32_t* n_rows,
+                  hipStream_t s) {
+  const int64_t want = (n + 65535) / 65536;
+  const int32_t g = static_cast<int32_t>(std::max<int64_t>(1, std::min<int64_t>(want, kLearnHistGrid)));
+  *n_rows = g;
+  lhist_kernel<<<g, kHistThreads, 0, s>>>(keys, n, tb.fb, tb.rows);
+  lreduce_kernel<<<kLearnBuckets / 256, 256, 0, s>>>(tb.rows, g, tb.hist);
+}
+
+void launch_lprefix(const LearnTables& tb, int32_t, int32_t fallback_tile, hipStream_t s) {
+  lprefix_kernel<<<1, 1024, 0, s>>>(tb, fallback_tile);
+}
+
+void launch_lpass_a(const uint32_t* in, uint32_t* out, int32_t n, const LearnTables& tb,
+                    uint64_t* flags, uint32_t* ctrl, hipStream_t s) {
+  if (n <= 0) return;
+  SegView sv{};
+  sv.n_seg = 1;
+  sv.n1 = n;
+  const int32_t tiles = (n + kLearnPassTile - 1) / kLearnPassTile;
+  onesweep_kernel<kLearnPassWaves, kLearnPassKpt, CoarseDigit>
+      <<<tiles, kLearnPassWaves * 64, 0, s>>>(in, out, sv, nullptr, nullptr, tb.hist_a, 0, flags,
+                                               ctrl, kStageBucket, CoarseDigit{tb.fb});
+}
+
+int32_t learn_pass_b_max_tiles(int32_t n) {
+  return (n + kLearnPassTile - 1) / kLearnPassTile + kLearnCoarse;
+}
+
+void launch_lpass_b(const uint32_t* in, uint32_t* out, int32_t n, const LearnTables& tb,
+                    uint64_t* flags, uint32_t* ctrl, hipStream_t s) {
+  if (n <= 0) return;
+  SegView sv{};
+  sv.n_seg = kLearnCoarse;
+  sv.start = tb.seg_start;
+  sv.n1 = n;
+  onesweep_kernel<kLearnPassWaves, kLearnPassKpt, FineDigit>
+      <<<learn_pass_b_max_tiles(n), kLearnPassWaves * 64, 0, s>>>(
+          in, out, sv, tb.tile_start, tb.tiles_b, tb.hist_b, 0, flags, ctrl, kStageBucket + 1,
+          FineDigit{tb.fb});
+}
+
+void launch_lsort(uint32_t* keys, const LearnTables& tb, hipStream_t s) {
+  lsort_kernel<4, 16><<<kLearnBuckets, 256, 0, s>>>(keys, tb.bstart, 1);
+  lsort_kernel<8, 16><<<kLearnBuckets, 512, 0, s>>>(keys, tb.bstart, kLearnSmallCap);
+  lsort_kernel<16, 16><<<kLearnBuckets, 1024, 0, s>>>(keys, tb.bstart, kLearnMidCap);
+  static_assert(16 * 64 * 16 == kLearnBigCap, "largest bucket-sort class");
+}
+
+void launch_lverify(const uint32_t* keys, const LearnTables& tb, int32_t n, uint32_t* ctrl,
+                    hipStream_t s) {
+  lverify_kernel<<<kLearnBuckets / 256, 256, 0, s>>>(keys, tb.bstart, n, ctrl);
+}
+
+void launch_lsample(const uint32_t* keys, int64_t, int32_t stride, uint32_t* ring,
+                    int64_t ring_cap, int64_t ring_pos, int64_t count, hipStream_t s) {
+  if (count <= 0) return;
+  lsample_kernel<<<static_cast<int>((count + 255) / 256), 256, 0, s>>>(keys, stride, ring,
+                                                                      ring_cap, ring_pos, count);
+}
+
+void launch_lmeta(const uint32_t* sorted, int32_t ns, uint32_t* meta, hipStream_t s) {
+  lmeta_kernel<<<1, 64, 0, s>>>(sorted, ns, meta);
+}
+
+void launch_ltrain_root(const uint32_t* sorted, int32_t ns, const uint32_t* meta, int32_t steps,
+                        float lr, float momentum, uint32_t seed, float* model, float* loss,
+                        hipStream_t s) {
+  ltrain_root_kernel<<<1, 1024, 0, s>>>(sorted, ns, meta, steps, lr, momentum, seed, model, loss);
+}
+
+void launch_lleaf_starts(const float* model, const uint32_t* meta, uint32_t* kleaf,
+                         hipStream_t s) {
+  lleaf_starts_kernel<<<(kCdfLeaves + 1 + 255) / 256, 256, 0, s>>>(model, meta, kleaf);
+}
+
+void launch_lfit_leaves(const uint32_t* sorted, int32_t ns, const uint32_t* kleaf, float* model,
+                        hipStream_t s) {
+  lfit_leaves_kernel<<<kCdfLeaves / 256, 256, 0, s>>>(sorted, ns, kleaf, model);
+}
+
+void launch_lboundaries(const float* model, const uint32_t* meta, const uint32_t* kleaf,
+                        uint32_t* fb, hipStream_t s) {
+  lboundaries_kernel<<<kLearnBuckets / 256, 256, 0, s>>>(model, meta, kleaf, fb);
+  lmonotone_kernel<<<1, 1024, 0, s>>>(fb);
+}
+
+void launch_llinear(float* model, hipStream_t s) { llinear_root_kernel<<<1, 64, 0, s>>>(model); }
+
+void launch_llinear_leaves(const uint32_t* kleaf, const uint32_t* meta, float* model,
+                           hipStream_t s) {
+  llinear_leaves_kernel<<<(kCdfLeaves + 1 + 255) / 256, 256, 0, s>>>(kleaf, meta, model);
+}
+
+void launch_lgrad(const float* xs, const float* ts, int32_t n, const float* model, float* grad,
+                  hipStream_t s) {
+  if (n > 0) lgrad_kernel<<<(n + 255) / 256, 256, 0, s>>>(xs, ts, n, model, grad);
+}
+
+void launch_lbucket(const uint32_t* keys, int32_t n, const float* model, const uint32_t* meta,
+                    const uint32_t* kleaf, const uint32_t* fb, int mode, uint32_t* q,
+                    hipStream_t s) {
+  if (n > 0) lbucket_kernel<<<(n + 255) / 256, 256, 0, s>>>(keys, n, model, meta, kleaf, fb, mode, q);
+}
+
+}  // namespace ocp

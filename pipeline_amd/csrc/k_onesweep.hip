@@ -104,4 +104,14 @@ void launch_histogram(const uint32_t* keys, const SegView& sv,
                                         morton_tile(tt), hist);
 }
 
+void launch_histogram_tiles(const uint32_t* keys, const SegView& sv,
+                            const int32_t* tile_start, int32_t total_tiles,
+                            int tile, uint32_t* hist, const TuneTable& tt,
+                            hipStream_t s) {
+  if (total_tiles <= 0) return;
+  const int grid = std::max(1, std::min(total_tiles, tt.morton_grid));
+  histogram_kernel<<<grid, 256, 0, s>>>(keys, sv, tile_start, total_tiles, tile,
+                                        hist);
+}
+
 }  // namespace ocp

@@ -245,7 +245,9 @@ __device__ __forceinline__ uint32_t wave_lookback(const uint64_t* flags, int32_t
 // ---------------------------------------------------------------- segments
 // A batch of independent clouds. `start` holds n_seg+1 point offsets; with
 // start == nullptr the batch is the single segment [0, n1) with box
-// (min1, range1). Per-segment boxes come from mn/rg when non-null.
+// (min1, range1). Per-segment boxes come from mn/rg when non-null. With
+// `end` non-null, segment k is [start[k], end[k]) and segments need not be
+// adjacent (the learned sort's oversized buckets).
 struct SegView {
   int32_t n_seg;
   const int32_t* start;
@@ -254,13 +256,14 @@ struct SegView {
   int32_t n1;
   float min1;
   float range1;
+  const int32_t* end;
 };
 
 __device__ __forceinline__ int32_t seg_begin(const SegView& s, int32_t k) {
   return s.start ? s.start[k] : 0;
 }
 __device__ __forceinline__ int32_t seg_end(const SegView& s, int32_t k) {
-  return s.start ? s.start[k + 1] : s.n1;
+  return s.end ? s.end[k] : (s.start ? s.start[k + 1] : s.n1);
 }
 __device__ __forceinline__ float seg_min(const SegView& s, int32_t k) {
   return s.mn ? s.mn[k] : s.min1;

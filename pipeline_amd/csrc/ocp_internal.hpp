@@ -27,7 +27,7 @@ enum Stage : uint32_t {
   kStagePass0 = 0,  // OneSweep passes 0..3
   kStageUnique = 4,
   kStageScan = 5,
-  kStageBucket = 6,
+  kStageBucket = 6,  // learned sort: 6 and 7
 };
 
 // Kernel configuration chosen by the autotuner (or the defaults).
@@ -72,6 +72,11 @@ void launch_morton(const float* pts, bool float3, uint32_t* keys,
 void launch_histogram(const uint32_t* keys, const SegView& sv,
                       const int32_t* tile_start, int32_t total_tiles,
                       uint32_t* hist, const TuneTable& tt, hipStream_t s);
+// Same, over tiles of `tile` keys (a tile map built for another tile size).
+void launch_histogram_tiles(const uint32_t* keys, const SegView& sv,
+                            const int32_t* tile_start, int32_t total_tiles,
+                            int tile, uint32_t* hist, const TuneTable& tt,
+                            hipStream_t s);
 void launch_onesweep_pass(const uint32_t* in, uint32_t* out,
                           const SegView& sv, const int32_t* tile_start,
                           int32_t total_tiles, const uint32_t* hist,

@@ -25,8 +25,12 @@
 
 namespace ocp {
 
+// A digit functor: setup(seg, tid, threads) runs once per tile before any
+// digit is taken (per-segment tables go to LDS there); operator() maps a key
+// to its digit in [0, 256).
 struct RadixDigit {
   uint32_t shift;
+  __device__ __forceinline__ void setup(int32_t, int, int) const {}
   __device__ __forceinline__ uint32_t operator()(uint32_t key) const {
     return (key >> shift) & 255u;
   }
@@ -56,10 +60,11 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
   const int tid = threadIdx.x;
   const uint32_t w = wave_id();
   const uint32_t lane = lane_id();
+  const TileRef r = find_tile(tile_start, sv.n_seg, t);
   for (int i = tid; i < WAVES * 256; i += THREADS) s_whist[i] = 0;
+  dig.setup(r.seg, tid, THREADS);
   __syncthreads();
   const uint32_t tag = ctrl[kCtrlEpoch] * 8u + stage;
-  const TileRef r = find_tile(tile_start, sv.n_seg, t);
   const int32_t sb = seg_begin(sv, r.seg);
   const int32_t base = sb + r.local * T;
   const int32_t cnt = min(T, seg_end(sv, r.seg) - base);
