@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kHistThreads) void lhist_kernel(const uint32_t* __r
   __syncthreads();
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kHistThreads;
   int64_t i = static_cast<int64_t>(blockIdx.x) * kHistThreads + tid;
-  constexpr int U = 4;
+  constexpr int U = 16;
   for (; i + (U - 1) * stride < n; i += U * stride) {
     uint32_t k[U];
 #pragma unroll

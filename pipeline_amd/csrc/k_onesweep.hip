@@ -12,28 +12,38 @@ namespace ocp {
 namespace {
 
 // Digit histograms of existing keys (the sort-only entry point; inside the
-// pipeline the Morton kernel produces them). Same persistent structure as
-// the Morton kernel.
+// pipeline the Morton kernel produces them). Persistent workgroups walk a
+// contiguous run of tiles; each thread first loads 16 keys (16 loads in
+// flight: one outstanding load per thread leaves HBM idle), then counts
+// them into LDS counters replicated 8 ways by lane, so that lanes of a wave
+// adding the same digit (the top byte of 30-bit keys has only 64 values)
+// rarely hit the same word.
+constexpr int kHistRep = 8;
+constexpr int kHistLoads = 16;
+
 __global__ __launch_bounds__(256) void histogram_kernel(
     const uint32_t* __restrict__ keys, SegView sv,
     const int32_t* __restrict__ tile_start, int32_t total_tiles, int tile,
     uint32_t* __restrict__ hist) {
-  __shared__ uint32_t s_hist[1024];
+  __shared__ uint32_t s_hist[1024 * kHistRep];
   const int tid = threadIdx.x;
+  const uint32_t rep = lane_id() & (kHistRep - 1);
   const int32_t per = (total_tiles + gridDim.x - 1) / gridDim.x;
   const int32_t t0 = blockIdx.x * per;
   const int32_t t1 = min(total_tiles, t0 + per);
-  for (int d = tid; d < 1024; d += 256) s_hist[d] = 0;
+  for (int d = tid; d < 1024 * kHistRep; d += 256) s_hist[d] = 0;
   __syncthreads();
   int32_t cur = -1;
   auto flush = [&](int32_t seg) {
     __syncthreads();
     for (int d = tid; d < 1024; d += 256) {
-      const uint32_t v = s_hist[d];
-      if (v) {
-        atomicAdd(&hist[static_cast<int64_t>(seg) * 1024 + d], v);
-        s_hist[d] = 0;
+      uint32_t v = 0;
+#pragma unroll
+      for (int r = 0; r < kHistRep; ++r) {
+        v += s_hist[d * kHistRep + r];
+        s_hist[d * kHistRep + r] = 0;
       }
+      if (v) atomicAdd(&hist[static_cast<int64_t>(seg) * 1024 + d], v);
     }
     __syncthreads();
   };
@@ -43,12 +53,23 @@ __global__ __launch_bounds__(256) void histogram_kernel(
     cur = r.seg;
     const int32_t base = seg_begin(sv, r.seg) + r.local * tile;
     const int32_t cnt = min(tile, seg_end(sv, r.seg) - base);
-    for (int32_t idx = tid; idx < cnt; idx += 256) {
-      const uint32_t key = keys[static_cast<int64_t>(base) + idx];
-      atomicAdd(&s_hist[key & 255u], 1u);
-      atomicAdd(&s_hist[256 + ((key >> 8) & 255u)], 1u);
-      atomicAdd(&s_hist[512 + ((key >> 16) & 255u)], 1u);
-      atomicAdd(&s_hist[768 + (key >> 24)], 1u);
+    const uint32_t* src = keys + base;
+    for (int32_t i0 = 0; i0 < cnt; i0 += 256 * kHistLoads) {
+      uint32_t k[kHistLoads];
+#pragma unroll
+      for (int u = 0; u < kHistLoads; ++u) {
+        const int32_t idx = i0 + u * 256 + tid;
+        k[u] = idx < cnt ? src[idx] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < kHistLoads; ++u) {
+        if (i0 + u * 256 + tid < cnt) {
+          atomicAdd(&s_hist[(k[u] & 255u) * kHistRep + rep], 1u);
+          atomicAdd(&s_hist[(256 + ((k[u] >> 8) & 255u)) * kHistRep + rep], 1u);
+          atomicAdd(&s_hist[(512 + ((k[u] >> 16) & 255u)) * kHistRep + rep], 1u);
+          atomicAdd(&s_hist[(768 + (k[u] >> 24)) * kHistRep + rep], 1u);
+        }
+      }
     }
   }
   if (cur >= 0) flush(cur);

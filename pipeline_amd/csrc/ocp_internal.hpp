@@ -33,7 +33,7 @@ enum Stage : uint32_t {
 // Kernel configuration chosen by the autotuner (or the defaults).
 struct TuneTable {
   int sort_waves = 8;      // OneSweep workgroup = sort_waves * 64 threads
-  int sort_kpt = 12;       // keys per thread in OneSweep
+  int sort_kpt = 16;       // keys per thread in OneSweep
   int morton_ppt = 16;     // points per thread in the Morton kernel
   int morton_grid = 1024;  // persistent Morton workgroups
   int scan_ipt = 16;       // items per thread in unique / edge-scan tiles

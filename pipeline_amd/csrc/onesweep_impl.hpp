@@ -12,7 +12,7 @@
 //      stable rank; the lowest lane bumps the wave's LDS digit counter.
 //   3. The tile's 256 digit counts are published at once (aggregate, or
 //      inclusive for a segment's first tile), then each of 256 threads looks
-//      back along its digit's chain, 16 tiles per round trip, with
+//      back along its digit's chain, kOneSweepWindow tiles per round trip, with
 //      agent-scope loads, and publishes the inclusive prefix.
 //   4. Keys (and their digits) are scattered into LDS in digit order, then
 //      written out with consecutive threads on consecutive LDS slots, so
@@ -24,6 +24,14 @@
 #include "ocp_internal.hpp"
 
 namespace ocp {
+
+#ifndef OCP_ONESWEEP_WINDOW
+#define OCP_ONESWEEP_WINDOW 4
+#endif
+// Predecessor tiles polled per lookback round trip. Every poll of an
+// unready window is wasted agent-scope traffic, so a short window measured
+// faster than 8, 16, 32 or 64 (2^26 keys: 1.44 ms vs 1.52, 1.85, 2.2, 3.3).
+constexpr int kOneSweepWindow = OCP_ONESWEEP_WINDOW;
 
 // A digit functor: setup(seg, tid, threads) runs once per tile before any
 // digit is taken (per-segment tables go to LDS there); operator() maps a key
@@ -130,7 +138,7 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
     uint32_t prev = 0;
     if (r.local > 0) {
       bool late = false;
-      prev = lookback_n<16>(flags + tid, 256, t, t - r.local, tag, &late);
+      prev = lookback_n<kOneSweepWindow>(flags + tid, 256, t, t - r.local, tag, &late);
       if (late) ctrl[kCtrlTimeout] = 1u;
       flag_store(&flags[static_cast<int64_t>(t) * 256 + tid],
                  flag_pack(tag, kFlagInc, prev + tile_cnt));
