@@ -10,13 +10,25 @@ edge count + scan -> octree). ``value`` is N / (time per build) in Mpoints/s.
 Also reported (same JSON line):
   roofline      the dominant kernel's algorithmic bytes per launch / its
                 average duration from HIP events on the context stream;
+                `traffic` is the HBM bytes per launch that rocprofv3's
+                FETCH_SIZE/WRITE_SIZE counters measured for that kernel on
+                this workload (profiles/traffic_plain_10M.json), when present;
   e2e_roofline  the whole build's algorithmic bytes (60N + 61U + 60K) / the
                 time per build, against 8 TB/s;
   kernels_ms    every kernel's average duration in eager builds;
   cpu_baseline  the CPU oracle (the reference CPU path restated, OpenMP) timed
-                on this host in the same run.
+                on this host in the same run;
+  batched       512 clouds (log-uniform 4096..2M points, uniform or clustered,
+                each with its own box) built by one segmented launch per stage
+                from a hipGraph, against 512 sequential plain builds (each
+                replayed from its own graph) of the same clouds;
+  learned_sort  2^26 clustered points over successive frames (rigid motion
+                <= 0.5 deg, <= 2 units): the CDF model is trained on 1% key
+                samples of the earlier frames and sorts frame t; OneSweep
+                sorts the same keys; both outputs are compared.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload plain|clustered]
+                  [--learned-frames F] [--no-batched] [--no-learned]
 """
 from __future__ import annotations
 
@@ -90,6 +102,112 @@ def cpu_baseline(pts: np.ndarray, budget_s: float = 20.0):
     }
 
 
+def load_traffic(kernel: str, n: int):
+    """Measured HBM bytes per launch of `kernel` for the default workload."""
+    path = os.path.join(ROOT, "profiles", "traffic_plain_10M.json")
+    if n != 10_000_000 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    v = table.get("kernels", {}).get(kernel)
+    return None if v is None else float(v)
+
+
+def bench_batched(ctx, pipeline_amd, clouds, steps: int, autotune: bool):
+    """512 clouds in one segmented build vs 512 sequential plain builds."""
+    pts, offs, smin, srg, kinds = clouds.batch(512, seed=114514)
+    sizes = offs[1:] - offs[:-1]
+    total = int(offs[-1])
+    d = ctx.upload(pts)
+    del pts
+    tuned = ctx.autotune(d, seg_offsets=offs, seg_min=smin, seg_range=srg, reps=2) \
+        if autotune else None
+    ctx.set_graph_mode(True)
+    ms_b = ctx.time_builds(d, 2, max(3, steps // 5), seg_offsets=offs, seg_min=smin,
+                           seg_range=srg)
+    u_b, k_b = ctx.counts(512)
+    # Sequential plain builds, largest first so that the workspace is sized
+    # once and every cloud's graph stays valid.
+    ms_seq = 0.0
+    same = True
+    for k in np.argsort(-sizes, kind="stable"):
+        ms_seq += ctx.time_builds(d, 1, 3, n=int(sizes[k]), min_coord=float(smin[k]),
+                                  range_=float(srg[k]), first=int(offs[k]))
+        u1, k1 = ctx.counts(1)
+        same &= bool(u1[0] == u_b[k] and k1[0] == k_b[k])
+    d.free()
+    thr_b = total / (ms_b * 1e-3) / 1e6
+    thr_s = total / (ms_seq * 1e-3) / 1e6
+    return {
+        "clouds": 512, "points": total, "min_cloud": int(sizes.min()),
+        "max_cloud": int(sizes.max()), "clustered_clouds": int(kinds.sum()),
+        "ms_batched": round(ms_b, 4), "ms_sequential": round(ms_seq, 4),
+        "batched_Mpts_s": round(thr_b, 2), "sequential_Mpts_s": round(thr_s, 2),
+        "speedup": round(thr_b / thr_s, 3), "counts_match_sequential": same,
+        "tune": tuned,
+    }
+
+
+def bench_learned(ctx, clouds, frames: int, n: int = 1 << 26, reps: int = 5):
+    """Learned-bucket sort vs OneSweep on frames of a moving clustered cloud."""
+    cur = clouds.clustered(n, seed=114514)
+    keys = []
+    for t in range(frames):
+        if t:
+            cur = clouds.rigid_motion(cur, seed=1000 + t)
+        dp = ctx.upload(cur)
+        k = ctx.empty(n, np.uint32)
+        ctx.k_ComputeMortonCode(dp, k, n, 0.0, 1024.0)
+        ctx.synchronize()
+        dp.free()
+        keys.append(k)
+    del cur
+    work = ctx.empty(n, np.uint32)
+    alt = ctx.empty(n, np.uint32)
+    ref = ctx.empty(n, np.uint32)
+
+    def timed(fn, src):
+        ts = []
+        for _ in range(reps + 1):
+            ctx.copy(work, src, 4 * n)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            ctx.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts[1:])) * 1e3
+
+    ctx.cdf_clear_samples()
+    per = []
+    for t in range(1, frames):
+        ctx.cdf_add_samples(keys[t - 1], n, 100)     # 1% samples of frame t-1
+        t0 = time.perf_counter()
+        loss = ctx.cdf_train(steps=2000, lr=1.0, momentum=0.9, seed=t)
+        train_ms = (time.perf_counter() - t0) * 1e3
+        ms_os = timed(lambda: ctx.k_SortKeys(work, alt, n), keys[t])
+        ctx.copy(ref, work, 4 * n)
+        st = {}
+        ms_l = timed(lambda: st.update(ctx.sort_learned(work, alt, n)), keys[t])
+        exact = bool(np.array_equal(work.to_host(), ref.to_host()))
+        per.append({"frame": t, "onesweep_ms": round(ms_os, 4), "learned_ms": round(ms_l, 4),
+                    "exact": exact, "train_ms": round(train_ms, 1), "loss": loss,
+                    "keys_fallback": st["keys_big"], "max_bucket": st["max_bucket"]})
+    ctx.cdf_reset(0, (1 << 30) - 1)
+    ms_lin = timed(lambda: ctx.sort_learned(work, alt, n), keys[-1])
+    for a in keys + [work, alt, ref]:
+        a.free()
+    os_ms = float(np.mean([p["onesweep_ms"] for p in per]))
+    l_ms = float(np.mean([p["learned_ms"] for p in per]))
+    return {
+        "n": n, "frames": frames, "motion": "rotation <= 0.5 deg about a random axis, "
+        "shift <= 2 units per axis, per frame", "samples": "every 100th key of each earlier frame",
+        "onesweep_ms": round(os_ms, 4), "learned_ms": round(l_ms, 4),
+        "speedup_vs_onesweep": round(os_ms / l_ms, 3),
+        "untrained_linear_cdf_ms": round(ms_lin, 4),
+        "all_exact": all(p["exact"] for p in per), "per_frame": per,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +219,10 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB device copy (traffic-counter reference)")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learned-frames", type=int, default=4,
+                    help="frames of the learned-sort comparison (30 for the full sequence)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -180,7 +302,8 @@ def main() -> int:
                    "autotuned": tuned is not None},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": load_traffic(dom, n),
                      "bytes_per_launch": dom_bytes, "avg_ms": round(kmean[dom], 5)},
         "e2e_roofline": {"achieved": round(e2e, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(e2e / HBM_PEAK_GBS, 4), "bytes": eb},
@@ -192,6 +315,15 @@ def main() -> int:
         cb_ok = cb["n_unique"] == u and cb["n_oct"] == k
         line["cpu_baseline"] = cb
         line["cpu_counts_match"] = bool(cb_ok)
+    d.free()
+    del pts
+    if not args.no_batched:
+        print("batched: 512 clouds", file=sys.stderr, flush=True)
+        line["batched"] = bench_batched(ctx, pipeline_amd, clouds, args.steps,
+                                        not args.no_autotune)
+    if not args.no_learned and args.learned_frames >= 2:
+        print(f"learned sort: {args.learned_frames} frames", file=sys.stderr, flush=True)
+        line["learned_sort"] = bench_learned(ctx, clouds, args.learned_frames)
     ctx.close()
     print(json.dumps(line))
     return 0

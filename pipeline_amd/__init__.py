@@ -438,8 +438,11 @@ class Context:
 
     def time_builds(self, points: DeviceArray, warmup: int, steps: int,
                     seg_offsets=None, seg_min=None, seg_range=None, n=None,
-                    min_coord=0.0, range_=1024.0) -> float:
+                    min_coord=0.0, range_=1024.0, first: int = 0) -> float:
+        """Mean ms per build over `steps` builds after `warmup` (plain when
+        seg_offsets is None: n points from point index `first`)."""
         f3 = int(points.shape[-1] == 3)
+        base = points.ptr + first * points.shape[-1] * 4
         ms = ctypes.c_float()
         if seg_offsets is None:
             n = points.shape[0] if n is None else n
@@ -452,8 +455,8 @@ class Context:
             mn = np.ascontiguousarray(seg_min, np.float32)
             rg = np.ascontiguousarray(seg_range, np.float32)
             n_seg = offs.size - 1
-        self._check(self.lib.ocp_time_builds(self.h, points.ptr, f3, _ptr(offs), _ptr(mn),
-                                             _ptr(rg), n_seg, warmup, steps,
+        self._check(self.lib.ocp_time_builds(self.h, ctypes.c_void_p(base), f3, _ptr(offs),
+                                             _ptr(mn), _ptr(rg), n_seg, warmup, steps,
                                              ctypes.byref(ms)))
         return float(ms.value)
 
