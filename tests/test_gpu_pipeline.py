@@ -101,3 +101,49 @@ def test_batched_matches_per_cloud_oracle(ctx, graph):
     finally:
         ctx.set_graph_mode(False)
         d.free()
+
+
+def test_plain_10M_uniform_full_oracle(ctx):
+    """The plain GPU workload at its real size: 10M uniform float4 points,
+    seed 114514, every stage output against the oracle."""
+    pts = clouds.uniform(10_000_000, seed=114514)
+    exp = oracle.build_plain(pts)
+    _check_segment(_plain(ctx, pts), 0, exp)
+
+
+def test_plain_2p24_clustered_full_oracle(ctx):
+    """The clustered workload at its real size (2^24 points, heavy duplicate
+    keys), every stage output against the oracle."""
+    pts = clouds.clustered(1 << 24, seed=114514)
+    exp = oracle.build_plain(pts)
+    assert exp["n_unique"] < (1 << 24) // 2  # the duplicate-heavy case
+    _check_segment(_plain(ctx, pts), 0, exp)
+
+
+def test_tuned_configs_give_identical_outputs(ctx):
+    """Every tile configuration the autotuner can pick gives the same bytes
+    (plain and batched)."""
+    pts = clouds.clustered(300000, seed=77)
+    exp = oracle.build_plain(pts)
+    bpts, offs, smin, srg, _ = _batched_case(6, 1000, 80000, seed=8)
+    d = ctx.upload(pts)
+    db = ctx.upload(bpts)
+    base = ctx.get_tune()
+    configs = ([dict(sort_waves=w, sort_kpt=k) for w, k in
+                ((8, 15), (4, 16), (4, 24), (8, 8), (8, 12), (8, 16), (16, 8))] +
+               [dict(scan_ipt=i) for i in (8, 16, 32)] +
+               [dict(morton_ppt=p, morton_grid=g) for p in (4, 8, 16, 32) for g in (512, 2048)])
+    try:
+        for cfg in configs:
+            ctx.set_tune(**{**base, **cfg})
+            ctx.build(d, pts.shape[0], 0.0, 1024.0)
+            _check_segment(ctx.fetch(), 0, exp)
+            ctx.build_batched(db, offs, smin, srg)
+            out = ctx.fetch(offs)
+            for s in (0, offs.size - 2):
+                seg = bpts[offs[s]:offs[s + 1]]
+                _check_segment(out, s, oracle.build_plain(seg, float(smin[s]), float(srg[s])))
+    finally:
+        ctx.set_tune(**base)
+        d.free()
+        db.free()
