@@ -92,8 +92,8 @@ struct View {
   int32_t lo;       // window [lo, lo + n) of nodes, [lo, lo + nc) of codes
   int32_t n;
   int32_t nc;
-  const v4i* rec;
-  const uint32_t* code;
+  lds_ptr<const v4i> rec;
+  lds_ptr<const uint32_t> code;
   __device__ __forceinline__ Rec R(int32_t q) const {
     if (static_cast<uint32_t>(q - lo) < static_cast<uint32_t>(n)) {
       const v4i r = rec[q - lo];
@@ -277,12 +277,9 @@ __device__ __forceinline__ void build_root(const View& v, int32_t root_level) {
     const uint32_t dn = delta(c0, v.CODE(n));
     const int32_t lo = 31 - static_cast<int32_t>(__clz(n));
     const int32_t lg = lo + ((1u << lo) < static_cast<uint32_t>(n));
-    const int64_t max_div = int64_t(1) << lg;
     int32_t s = 0;
-    int64_t div = 2;
-    for (uint32_t t = (static_cast<uint32_t>(n) + 1u) / 2u; div <= max_div;
-         div <<= 1, t = static_cast<uint32_t>((n + div - 1) / div)) {
-      const int32_t st = s + static_cast<int32_t>(t);
+    for (int32_t k = 1; k <= lg; ++k) {  // step ceil(n / 2^k), as a shift
+      const int32_t st = s + static_cast<int32_t>((static_cast<uint32_t>(n) + (1u << k) - 1u) >> k);
       if (st <= n && delta(c0, v.CODE(st)) > dn) s = st;
     }
     int32_t mem[2] = {-1, -1};
@@ -447,7 +444,7 @@ __global__ __launch_bounds__(kThreads) void oct_build_kernel(
   __shared__ int32_t s_emax;
 
   if (a.ctrl && a.ctrl[kCtrlOctOk] == 0) return;
-  const TileRef r = find_tile(tile_start, sv.n_seg, blockIdx.x);
+  const TileRef r = find_tile(tile_start, sv.n_seg, xcd_tile(blockIdx.x, gridDim.x));
   SegBind b;
   b.m = a.n_unique ? a.n_unique[r.seg] - 1 : a.n_brt_host;
   const int32_t i0 = r.local * kNodeTile;
@@ -491,7 +488,7 @@ __global__ __launch_bounds__(kThreads) void oct_build_kernel(
   emax = wave_incl_max(emax);
   if (lane_id() == 0) atomicMax(&s_emax, emax);
   __syncthreads();
-  const View v{b, lo, wn, wnc, s_rec, s_code};
+  const View v{b, lo, wn, wnc, as_lds<const v4i>(s_rec), as_lds<const uint32_t>(s_code)};
   // Compact the block's work: the root and the non-empty chains.
 #pragma unroll
   for (int h = 0; h < kNodeTile / kThreads; ++h) {

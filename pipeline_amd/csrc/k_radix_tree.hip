@@ -34,15 +34,12 @@ __device__ __forceinline__ int32_t log2_ceil(uint32_t x) {
   const int32_t lo = 31 - static_cast<int32_t>(clz32(x));
   return lo + ((1u << lo) < x);
 }
-__device__ __forceinline__ uint32_t ceil_div(uint32_t a, uint32_t b) {
-  return (a + b - 1) / b;
-}
 
 // Codes of a segment, served from an LDS window [lo, lo + n) when the
 // index falls inside it. Most searches of a node stay close to the node.
 struct Codes {
   const uint32_t* g;
-  const uint32_t* w;
+  lds_ptr<const uint32_t> w;
   int32_t lo;
   int32_t n;
   __device__ __forceinline__ uint32_t operator[](int32_t q) const {
@@ -50,52 +47,66 @@ struct Codes {
   }
 };
 
-constexpr int kHalo = 256;
+#ifndef OCP_RADIX_HALO
+#define OCP_RADIX_HALO 256
+#endif
+constexpr int kHalo = OCP_RADIX_HALO;  // codes staged on either side of a tile
 
-// Split search of 04_radix_tree.cpp:94-104 for node i with far end j.
+// Split search of 04_radix_tree.cpp:94-104 for node i with far end j. The
+// reference's divisor runs over 2, 4, ..., 2^log2_ceil(l) and its step is
+// ceil(l / divisor); with power-of-two divisors that is a shift, so no
+// integer division is left on the device.
 __device__ __forceinline__ int32_t split_search(const Codes& codes,
                                                 int32_t i, int32_t d,
                                                 int32_t l, int32_t n,
                                                 uint32_t delta_node) {
   const uint32_t code_i = codes[i];
   int32_t s = 0;
-  const int64_t max_divisor = int64_t(1) << log2_ceil(static_cast<uint32_t>(l));
+  const int32_t lg = log2_ceil(static_cast<uint32_t>(l));
   const int32_t s_cutoff = (d == -1) ? i - 1 : n - i - 1;
-  int64_t divisor = 2;
-  for (uint32_t t = ceil_div(static_cast<uint32_t>(l), 2); divisor <= max_divisor;
-       divisor <<= 1, t = ceil_div(static_cast<uint32_t>(l), static_cast<uint32_t>(divisor))) {
-    const int32_t st = s + static_cast<int32_t>(t);
+  for (int32_t k = 1; k <= lg; ++k) {
+    const int32_t t = static_cast<int32_t>((static_cast<uint32_t>(l) + (1u << k) - 1u) >> k);
+    const int32_t st = s + t;
     if (st <= s_cutoff && delta(code_i, codes[i + st * d]) > delta_node) s = st;
   }
   return i + s * d + min(d, 0);
 }
 
-__global__ __launch_bounds__(256) void radix_tree_kernel(
-    const uint32_t* __restrict__ codes_all, uint8_t* __restrict__ prefix_all,
-    uint8_t* __restrict__ hl_all, uint8_t* __restrict__ hr_all,
-    int32_t* __restrict__ lc_all, int32_t* __restrict__ par_all, SegView sv,
-    const int32_t* __restrict__ tile_start, const int32_t* __restrict__ n_unique,
-    int32_t n_brt_host) {
-  __shared__ uint32_t s_codes[256 + 2 * kHalo];
-  const TileRef r = find_tile(tile_start, sv.n_seg, blockIdx.x);
-  const int32_t n = n_unique ? n_unique[r.seg] - 1 : n_brt_host;
-  const int32_t i0 = r.local * 256;
-  if (i0 >= n) return;
-  const int32_t sb = seg_begin(sv, r.seg);
-  int32_t* parent = par_all + sb;
-  // Stage codes [i0 - halo, i0 + 256 + halo) of the n+1 codes.
-  const int32_t lo = max(0, i0 - kHalo);
-  const int32_t hi = min(n + 1, i0 + 256 + kHalo);
-  for (int32_t q = lo + static_cast<int32_t>(threadIdx.x); q < hi; q += 256)
-    s_codes[q - lo] = codes_all[sb + q];
-  __syncthreads();
-  const Codes codes{codes_all + sb, s_codes, lo, hi - lo};
-  const int32_t i = i0 + static_cast<int32_t>(threadIdx.x);
-  if (i >= n) return;
+// Wide nodes. All three searches of a node look for the largest x with a
+// predicate that is monotone in x (codes are sorted, so the common prefix
+// with code i only shrinks away from i): the reference's exponential search
+// stops at the first power of two that fails, its binary searches return
+// the largest x below their bound that holds (for the length: x < l_max and
+// x <= l_cutoff; for the split: x <= min(l - 1, s_cutoff), the predicate
+// failing at l). A whole wave evaluates 64 candidates per round trip
+// instead of one, so a node spanning 2^k keys costs about k/6 dependent
+// loads instead of 3k.
+template <class Pred>
+__device__ __forceinline__ int32_t wave_max_true(int32_t lo, int32_t hi, Pred pred) {
+  // largest x in [lo, hi) with pred(x); pred(lo) holds.
+  const int32_t lane = static_cast<int32_t>(lane_id());
+  while (hi - lo > 1) {
+    const int32_t step = (hi - lo + 63) >> 6;
+    const int32_t x = lo + step * (lane + 1);
+    const bool ok = x < hi && pred(x);
+    const int32_t c = __popcll(__ballot(ok));
+    const int32_t nlo = lo + step * c;
+    hi = min(hi, lo + step * (c + 1));
+    lo = nlo;
+  }
+  return lo;
+}
 
+struct NodeOut {
+  uint32_t delta_node;
+  int32_t gamma;
+  int32_t j;
+};
+
+__device__ __forceinline__ NodeOut wave_node(const Codes& codes, int32_t i, int32_t n) {
+  const int32_t lane = static_cast<int32_t>(lane_id());
   const uint32_t code_i = codes[i];
-  int32_t d;
-  int32_t l = 0;
+  int32_t d, l;
   if (i == 0) {
     d = 1;
     l = n - 1;
@@ -104,37 +115,131 @@ __global__ __launch_bounds__(256) void radix_tree_kernel(
                          static_cast<int32_t>(delta(code_i, codes[i - 1]));
     d = (diff > 0) - (diff < 0);
     const uint32_t delta_min = delta(code_i, codes[i - d]);
-    int64_t l_max = 2;
-    while (i + l_max * d >= 0 && i + l_max * d <= n &&
-           delta(code_i, codes[i + l_max * d]) > delta_min &&
-           l_max < (int64_t(1) << 31))
-      l_max *= 2;
+    // exponential search: lane k tests l_max = 2^k, k = 1..30
+    bool ok = false;
+    if (lane >= 1 && lane <= 30) {
+      const int64_t q = static_cast<int64_t>(i) + (int64_t(1) << lane) * d;
+      ok = q >= 0 && q <= n && delta(code_i, codes[static_cast<int32_t>(q)]) > delta_min;
+    }
+    const int32_t c = __popcll(__ballot(ok));
+    const int32_t l_max = 1 << (c + 1);
     const int32_t l_cutoff = (d == -1) ? i : n - i;
-    for (int64_t t = l_max / 2, divisor = 2; t >= 1; divisor *= 2, t = l_max / divisor) {
-      if (l + t <= l_cutoff && delta(code_i, codes[i + (l + t) * d]) > delta_min)
-        l += static_cast<int32_t>(t);
+    l = wave_max_true(0, min(l_max, l_cutoff + 1), [&](int32_t x) {
+      return delta(code_i, codes[i + x * d]) > delta_min;
+    });
+  }
+  NodeOut o;
+  o.j = i + l * d;
+  o.delta_node = delta(code_i, codes[o.j]);
+  const int32_t s_cutoff = (d == -1) ? i - 1 : n - i - 1;
+  const int32_t s = l <= 0 ? 0 : wave_max_true(0, min(l - 1, s_cutoff) + 1, [&](int32_t x) {
+    return delta(code_i, codes[i + x * d]) > o.delta_node;
+  });
+  o.gamma = i + s * d + min(d, 0);
+  return o;
+}
+
+// Nodes whose exponential search is still running at this length are
+// finished by a whole wave; the rest stay inside the LDS window.
+constexpr int32_t kLocalMax = kHalo / 2;
+
+__global__ __launch_bounds__(256) void radix_tree_kernel(
+    const uint32_t* __restrict__ codes_all, uint8_t* __restrict__ prefix_all,
+    uint8_t* __restrict__ hl_all, uint8_t* __restrict__ hr_all,
+    int32_t* __restrict__ lc_all, int32_t* __restrict__ par_all, SegView sv,
+    const int32_t* __restrict__ tile_start, const int32_t* __restrict__ n_unique,
+    int32_t n_brt_host) {
+  static_assert(kLocalMax <= kHalo / 2 + 1, "local searches must stay in the window");
+  __shared__ uint32_t s_codes[kNodeTile + 2 * kHalo];
+  __shared__ int32_t s_wide[kNodeTile];
+  __shared__ uint32_t s_nwide;
+  const TileRef r = find_tile(tile_start, sv.n_seg, xcd_tile(blockIdx.x, gridDim.x));
+  const int32_t n = n_unique ? n_unique[r.seg] - 1 : n_brt_host;
+  const int32_t i0 = r.local * kNodeTile;
+  if (i0 >= n) return;
+  const int32_t sb = seg_begin(sv, r.seg);
+  int32_t* parent = par_all + sb;
+  if (threadIdx.x == 0) s_nwide = 0;
+  // Stage codes [i0 - halo, i0 + kNodeTile + halo) of the n+1 codes.
+  const int32_t lo = max(0, i0 - kHalo);
+  const int32_t hi = min(n + 1, i0 + kNodeTile + kHalo);
+  for (int32_t q = lo + static_cast<int32_t>(threadIdx.x); q < hi; q += 256)
+    s_codes[q - lo] = codes_all[sb + q];
+  __syncthreads();
+  const Codes codes{codes_all + sb, as_lds<const uint32_t>(s_codes), lo, hi - lo};
+
+  auto emit = [&](int32_t node, uint32_t delta_node, int32_t gamma, int32_t j) {
+    const bool leaf_l = min(node, j) == gamma;
+    const bool leaf_r = max(node, j) == gamma + 1;
+    prefix_all[sb + node] = static_cast<uint8_t>(delta_node);
+    hl_all[sb + node] = leaf_l;
+    hr_all[sb + node] = leaf_r;
+    lc_all[sb + node] = gamma;
+    if (node > 0) {
+      if (!leaf_l) parent[gamma] = node;
+      if (!leaf_r) parent[gamma + 1] = node;
+    }
+  };
+
+  for (int32_t k = 0; k < kNodeTile / 256; ++k) {
+  const int32_t i = i0 + k * 256 + static_cast<int32_t>(threadIdx.x);
+  bool wide = false;
+  if (i < n) {
+    if (i == 0) {
+      wide = true;
+    } else {
+      const uint32_t code_i = codes[i];
+      const int32_t diff = static_cast<int32_t>(delta(code_i, codes[i + 1])) -
+                           static_cast<int32_t>(delta(code_i, codes[i - 1]));
+      const int32_t d = (diff > 0) - (diff < 0);
+      const uint32_t delta_min = delta(code_i, codes[i - d]);
+      // Exponential search for an upper bound l_max (a power of two), then
+      // binary search of the length: t = l_max / 2, l_max / 4, ..., 1.
+      int32_t l_max = 2;
+      while (l_max <= kLocalMax && i + l_max * d >= 0 && i + l_max * d <= n &&
+             delta(code_i, codes[i + l_max * d]) > delta_min)
+        l_max *= 2;
+      if (l_max > kLocalMax) {
+        wide = true;
+      } else {
+        const int32_t l_cutoff = (d == -1) ? i : n - i;
+        int32_t l = 0;
+        for (int32_t t = l_max >> 1; t >= 1; t >>= 1) {
+          if (l + t <= l_cutoff && delta(code_i, codes[i + (l + t) * d]) > delta_min) l += t;
+        }
+        const int32_t j = i + l * d;
+        const uint32_t delta_node = delta(code_i, codes[j]);
+        emit(i, delta_node, split_search(codes, i, d, l, n, delta_node), j);
+      }
     }
   }
-  const int32_t j = i + l * d;
-  const uint32_t delta_node = delta(code_i, codes[j]);
-  const int32_t gamma = split_search(codes, i, d, l, n, delta_node);
-  const bool leaf_l = min(i, j) == gamma;
-  const bool leaf_r = max(i, j) == gamma + 1;
-  prefix_all[sb + i] = static_cast<uint8_t>(delta_node);
-  hl_all[sb + i] = leaf_l;
-  hr_all[sb + i] = leaf_r;
-  lc_all[sb + i] = gamma;
-  if (i > 0) {
-    if (!leaf_l) parent[gamma] = i;
-    if (!leaf_r) parent[gamma + 1] = i;
-  } else {
-    // The true root spans [0, n]: its internal children have no other
-    // writer, and sequentially they end up holding node 0.
-    parent[0] = 0;
-    const uint32_t dn = delta(code_i, codes[n]);
-    const int32_t g = split_search(codes, 0, 1, n, n + 1, dn);
-    if (g != 0 && g < n) parent[g] = 0;
-    if (g + 1 != n && g + 1 < n) parent[g + 1] = 0;
+  const uint64_t bal = __ballot(wide);
+  if (bal) {
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(&s_nwide, static_cast<uint32_t>(__popcll(bal)));
+    base = __shfl(base, 0, 64);
+    if (wide) s_wide[base + __popcll(bal & lanemask_lt())] = i;
+  }
+  }
+  __syncthreads();
+  const uint32_t nw = s_nwide;
+  for (uint32_t q = wave_id(); q < nw; q += 4) {
+    const int32_t node = s_wide[q];
+    const NodeOut o = wave_node(codes, node, n);
+    if (lane_id() == 0) emit(node, o.delta_node, o.gamma, o.j);
+    if (node == 0) {
+      // The true root spans [0, n]: its internal children have no other
+      // writer, and sequentially they end up holding node 0. Its split is
+      // the largest s <= n - 1 with delta(c0, codes[s]) > delta(c0, codes[n]).
+      const uint32_t c0 = codes[0];
+      const uint32_t dn = delta(c0, codes[n]);
+      const int32_t g0 = wave_max_true(0, n, [&](int32_t q) { return delta(c0, codes[q]) > dn; });
+      if (lane_id() == 0) {
+        parent[0] = 0;
+        if (g0 != 0 && g0 < n) parent[g0] = 0;
+        if (g0 + 1 != n && g0 + 1 < n) parent[g0 + 1] = 0;
+      }
+    }
   }
 }
 
@@ -147,6 +252,7 @@ void launch_radix_tree(const uint32_t* codes, uint8_t* prefix_n,
                        const int32_t* n_unique, int32_t n_brt_host,
                        hipStream_t s) {
   if (total_tiles <= 0) return;
+  static_assert(kNodeTile % 256 == 0, "whole node rows per thread");
   radix_tree_kernel<<<total_tiles, 256, 0, s>>>(
       codes, prefix_n, has_leaf_left, has_leaf_right, left_child, parent, sv,
       tile_start, n_unique, n_brt_host);

@@ -22,6 +22,19 @@ namespace ocp {
 constexpr int kWave = 64;
 constexpr int kMortonBits = 30;
 
+// ---------------------------------------------------------------- LDS pointers
+// Pointers typed as LDS (address space 3). A window accessor such as
+// `in_window ? lds[q] : global[q]` through generic pointers lets the
+// compiler fold both loads into one FLAT load, which goes through the vector
+// memory path even for LDS; with the LDS pointer typed, the window hit is a
+// ds_read.
+template <class T>
+using lds_ptr = __attribute__((address_space(3))) T*;
+template <class T>
+__device__ __forceinline__ lds_ptr<T> as_lds(T* p) {
+  return (lds_ptr<T>)(p);
+}
+
 // ---------------------------------------------------------------- lanes
 __device__ __forceinline__ uint32_t lane_id() {
   return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
@@ -287,6 +300,17 @@ __device__ __forceinline__ TileRef find_tile(const int32_t* tile_start,
     if (tile_start[mid] <= t) lo = mid; else hi = mid;
   }
   return {lo, t - tile_start[lo]};
+}
+
+// Workgroups are dispatched round-robin over the chip's 8 XCDs (block b to
+// XCD b % 8), and each XCD has its own L2. For kernels without cross-tile
+// waits, this maps block b to a tile so that every XCD walks one contiguous
+// run of tiles in order: the halo a tile shares with its neighbour is then
+// still in that XCD's L2. Bijective on [0, g).
+__device__ __forceinline__ int32_t xcd_tile(int32_t b, int32_t g) {
+  const int32_t x = b & 7, k = b >> 3;
+  const int32_t per = g >> 3, rem = g & 7;
+  return x * per + min(x, rem) + k;
 }
 
 // Reference OctNode layout (include/shared/types.h:28-34), 60 bytes.

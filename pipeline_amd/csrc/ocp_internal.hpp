@@ -42,7 +42,7 @@ struct TuneTable {
 inline int sort_tile(const TuneTable& t) { return t.sort_waves * 64 * t.sort_kpt; }
 inline int morton_tile(const TuneTable& t) { return 256 * t.morton_ppt; }
 inline int scan_tile(const TuneTable& t) { return 256 * t.scan_ipt; }
-constexpr int kNodeTile = 256;  // one thread per BRT node
+constexpr int kNodeTile = 256;  // BRT nodes per radix-tree / octree tile
 
 bool sort_config_supported(int waves, int kpt);
 bool scan_config_supported(int ipt);
