@@ -53,7 +53,10 @@ namespace {
 
 constexpr int32_t kMaxEdge = 10;
 constexpr int kThreads = 128;
-constexpr int kHalo = 64;
+#ifndef OCP_OCT_HALO
+#define OCP_OCT_HALO 64
+#endif
+constexpr int kHalo = OCP_OCT_HALO;  // BRT nodes staged on either side of a tile
 constexpr int kWinN = kNodeTile + 2 * kHalo;
 constexpr uint32_t kLeafL = 1u << 30;
 constexpr uint32_t kLeafR = 1u << 31;
@@ -473,16 +476,36 @@ __global__ __launch_bounds__(kThreads) void oct_build_kernel(
     s_emax = 0;
   }
   __syncthreads();
+  // All of the window's loads are issued before any is used: one round
+  // trip to memory instead of one per row of the window.
+  constexpr int kRows = (kWinN + kThreads - 1) / kThreads;
   int32_t emax = 0;
-  for (int q = tid; q < wnc; q += kThreads) {
+  uint32_t r_code[kRows];
+  int32_t r_p[kRows], r_e[kRows], r_lc[kRows];
+  uint8_t r_hl[kRows], r_hr[kRows], r_pn[kRows];
+#pragma unroll
+  for (int k = 0; k < kRows; ++k) {
+    const int32_t q = tid + k * kThreads;
     const int32_t g = lo + q;
-    s_code[q] = b.codes[g];
+    if (q < wnc) r_code[k] = b.codes[g];
     if (q < wn) {
-      const int32_t e = b.e[g];
-      const uint32_t l = static_cast<uint32_t>(b.lc[g]) | (b.hl[g] ? kLeafL : 0u) |
-                         (b.hr[g] ? kLeafR : 0u);
-      s_rec[q] = v4i{b.p[g], e, static_cast<int32_t>(l), static_cast<int32_t>(b.pn[g])};
-      emax = max(emax, e);
+      r_p[k] = b.p[g];
+      r_e[k] = b.e[g];
+      r_lc[k] = b.lc[g];
+      r_hl[k] = b.hl[g];
+      r_hr[k] = b.hr[g];
+      r_pn[k] = b.pn[g];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kRows; ++k) {
+    const int32_t q = tid + k * kThreads;
+    if (q < wnc) s_code[q] = r_code[k];
+    if (q < wn) {
+      const uint32_t l = static_cast<uint32_t>(r_lc[k]) | (r_hl[k] ? kLeafL : 0u) |
+                         (r_hr[k] ? kLeafR : 0u);
+      s_rec[q] = v4i{r_p[k], r_e[k], static_cast<int32_t>(l), static_cast<int32_t>(r_pn[k])};
+      emax = max(emax, r_e[k]);
     }
   }
   emax = wave_incl_max(emax);

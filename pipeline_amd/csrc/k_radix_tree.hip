@@ -163,8 +163,21 @@ __global__ __launch_bounds__(256) void radix_tree_kernel(
   // Stage codes [i0 - halo, i0 + kNodeTile + halo) of the n+1 codes.
   const int32_t lo = max(0, i0 - kHalo);
   const int32_t hi = min(n + 1, i0 + kNodeTile + kHalo);
-  for (int32_t q = lo + static_cast<int32_t>(threadIdx.x); q < hi; q += 256)
-    s_codes[q - lo] = codes_all[sb + q];
+  {
+    // every load of the window in flight at once
+    constexpr int kRows = (kNodeTile + 2 * kHalo + 255) / 256;
+    uint32_t rc[kRows];
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) {
+      const int32_t q = lo + static_cast<int32_t>(threadIdx.x) + k * 256;
+      if (q < hi) rc[k] = codes_all[sb + q];
+    }
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) {
+      const int32_t q = lo + static_cast<int32_t>(threadIdx.x) + k * 256;
+      if (q < hi) s_codes[q - lo] = rc[k];
+    }
+  }
   __syncthreads();
   const Codes codes{codes_all + sb, as_lds<const uint32_t>(s_codes), lo, hi - lo};
 
