@@ -294,7 +294,8 @@ int ocp_cdf_load(ocp_ctx ctx, const char* path);
  * respect to the 49 root parameters, into a host array. */
 int ocp_cdf_grad(ocp_ctx ctx, const float* xs, const float* ts, int32_t n, float* grad);
 /* Bucket of each key (device arrays): mode 0 evaluates the model, mode 1
- * the compiled boundaries (the two agree). */
+ * searches the compiled boundaries, mode 2 goes through the cell table as
+ * the sort does (all three agree). */
 int ocp_cdf_buckets(ocp_ctx ctx, const uint32_t* keys, uint32_t n, int mode, uint32_t* q);
 /* Host copy of the 16384 boundaries: bucket(key) = #{f >= 1 : fb[f] <= key}. */
 int ocp_cdf_boundaries(ocp_ctx ctx, uint32_t* fb);

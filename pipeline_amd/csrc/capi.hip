@@ -91,6 +91,7 @@ struct LearnState {
   uint32_t* meta = nullptr;   // [kmin, kmax, 0, 0]
   uint32_t* kleaf = nullptr;  // kCdfLeaves + 1
   uint32_t* fb = nullptr;     // kLearnBuckets
+  uint16_t* ctab = nullptr;   // kLearnCells + 2
   float* loss = nullptr;
   bool compiled = false;
   uint32_t* ring = nullptr;   // sample ring
@@ -549,6 +550,7 @@ int check_ctx(ocp_ctx c) { return c ? OCP_OK : OCP_ERR_INVALID; }
 
 void free_learn(LearnState& L) {
   dev_free(L.model); dev_free(L.meta); dev_free(L.kleaf); dev_free(L.fb); dev_free(L.loss);
+  dev_free(L.ctab);
   dev_free(L.ring); dev_free(L.samp_a); dev_free(L.samp_b);
   LearnTables& t = L.tb;
   dev_free(t.rows); dev_free(t.hist); dev_free(t.bstart); dev_free(t.hist_a);
@@ -1099,6 +1101,7 @@ static int learn_model_alloc(ocp_ctx c) {
   int r = 0;
   if ((r = dev_alloc(c, &L.model, kCdfParams)) || (r = dev_alloc(c, &L.meta, 4)) ||
       (r = dev_alloc(c, &L.kleaf, kCdfLeaves + 1)) || (r = dev_alloc(c, &L.fb, kLearnBuckets)) ||
+      (r = dev_alloc(c, &L.ctab, kLearnCells + 2)) ||
       (r = dev_alloc(c, &L.loss, 1)))
     return r;
   OCP_HIP(c, hipMemsetAsync(L.model, 0, sizeof(float) * kCdfParams, c->stream));
@@ -1140,6 +1143,8 @@ static int learn_tables_alloc(ocp_ctx c, int64_t n) {
     L.n_cap = cap;
   }
   t.fb = L.fb;
+  t.ctab = L.ctab;
+  t.meta = L.meta;
   return OCP_OK;
 }
 
@@ -1151,7 +1156,7 @@ static int learn_compile(ocp_ctx c, bool fit_from_samples, bool linear_leaves) {
   if (fit_from_samples)
     launch_lfit_leaves(L.samp_a, static_cast<int32_t>(L.ring_fill), L.kleaf, L.model, c->stream);
   if (linear_leaves) launch_llinear_leaves(L.kleaf, L.meta, L.model, c->stream);
-  launch_lboundaries(L.model, L.meta, L.kleaf, L.fb, c->stream);
+  launch_lboundaries(L.model, L.meta, L.kleaf, L.fb, L.ctab, c->stream);
   OCP_CHECK_LAUNCH(c);
   L.compiled = true;
   return OCP_OK;
@@ -1319,11 +1324,12 @@ int ocp_cdf_grad(ocp_ctx c, const float* xs, const float* ts, int32_t n, float* 
 int ocp_cdf_buckets(ocp_ctx c, const uint32_t* keys, uint32_t n, int mode, uint32_t* q) {
   if (check_ctx(c)) return OCP_ERR_INVALID;
   if (n > 0 && (!keys || !q)) return fail(c, OCP_ERR_INVALID, "null pointer");
-  if (mode != 0 && mode != 1) return fail(c, OCP_ERR_INVALID, "mode must be 0 or 1");
+  if (mode < 0 || mode > 2) return fail(c, OCP_ERR_INVALID, "mode must be 0, 1 or 2");
   if (n > kMaxPoints) return fail(c, OCP_ERR_CAPACITY, "n > 2^30-1");
   LearnState& L = c->learn;
   if (!L.compiled) return fail(c, OCP_ERR_STATE, "no CDF model: train, set, reset or load one");
-  launch_lbucket(keys, static_cast<int32_t>(n), L.model, L.meta, L.kleaf, L.fb, mode, q, c->stream);
+  launch_lbucket(keys, static_cast<int32_t>(n), L.model, L.meta, L.kleaf, L.fb, L.ctab, mode, q,
+                 c->stream);
   OCP_CHECK_LAUNCH(c);
   return OCP_OK;
 }

@@ -15,8 +15,10 @@
 // q is non-decreasing in the key, so it is fully described by its 16384
 // bucket boundaries fb[f] = min{key : q(key) >= f}: q(key) = #{f >= 1 :
 // fb[f] <= key}. The compile step finds every fb[f] by binary search over
-// the 32-bit key space with the model itself; the sort evaluates q only
-// through fb (a binary search in LDS), never the network.
+// the 32-bit key space with the model itself, and a table of the bucket at
+// each of 4096 equal cells of the key range; the sort evaluates q only
+// through those (one LDS read for most keys, a short search in LDS for keys
+// in cells that hold a boundary), never the network.
 //
 // Sort (keys only, so stability is irrelevant and any correct per-bucket
 // sort gives the bytes of np.sort):
@@ -93,57 +95,106 @@ __device__ __forceinline__ uint32_t q_model(const CdfDev& v, uint32_t key) {
   return s >= static_cast<float>(kLearnBuckets) ? kLearnBuckets - 1u : static_cast<uint32_t>(s);
 }
 
-// q through the boundaries, by branch-free binary search: `cb` holds
-// fb[64 c] for c < 256, `fbs` the 64 boundaries of one coarse segment.
-__device__ __forceinline__ uint32_t coarse_of(const uint32_t* cb, uint32_t key) {
-  uint32_t c = 0;
-#pragma unroll
-  for (uint32_t step = 128; step > 0; step >>= 1) c += cb[c + step] <= key ? step : 0u;
-  return c;
+// Bounds [lo, hi] on q(key) from the cell table (q is non-decreasing, so a
+// key's bucket lies between those of its cell's first key and the next
+// cell's); keys outside the table's range get the open-ended bounds.
+template <class P16>
+__device__ __forceinline__ void cell_bounds(P16 ctab, uint32_t kmin, uint32_t cshift,
+                                            uint32_t key, uint32_t& lo, uint32_t& hi) {
+  if (key < kmin) {
+    lo = 0;
+    hi = ctab[0];
+    return;
+  }
+  const uint32_t c = (key - kmin) >> cshift;
+  if (c >= static_cast<uint32_t>(kLearnCells)) {
+    lo = ctab[kLearnCells];
+    hi = kLearnBuckets - 1;
+    return;
+  }
+  lo = ctab[c];
+  hi = ctab[c + 1];
 }
-__device__ __forceinline__ uint32_t fine_of(const uint32_t* fbs, uint32_t key) {
-  uint32_t f = 0;
-#pragma unroll
-  for (uint32_t step = 32; step > 0; step >>= 1) f += fbs[f + step] <= key ? step : 0u;
-  return f;
+// Largest f in [lo, hi] with tab[f - base] <= key, tab[lo - base] <= key.
+template <class P32>
+__device__ __forceinline__ uint32_t search_range(P32 tab, uint32_t base, uint32_t lo,
+                                                 uint32_t hi, uint32_t key) {
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (tab[mid - base] <= key) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
 }
 
 // Digit functors of the two scatter passes; their tables live in LDS.
+__shared__ uint16_t s_ctab[kLearnCells + 2];
 __shared__ uint32_t s_cb[kLearnCoarse];
 __shared__ uint32_t s_fbs[kLearnFine];
+__shared__ uint32_t s_kmin, s_cshift, s_seg_id;
+
+__device__ __forceinline__ void load_cells(const LearnTables& tb, int tid, int threads) {
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(tb.ctab);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(s_ctab);
+  for (int i = tid; i < (kLearnCells + 2) / 2; i += threads) dst[i] = src[i];
+  if (tid == 0) {
+    s_kmin = tb.meta[0];
+    s_cshift = tb.meta[2];
+  }
+}
 
 struct CoarseDigit {
-  const uint32_t* fb;
+  LearnTables tb;
   __device__ __forceinline__ void setup(int32_t, int tid, int threads) const {
-    for (int i = tid; i < kLearnCoarse; i += threads) s_cb[i] = fb[i * kLearnFine];
+    load_cells(tb, tid, threads);
+    for (int i = tid; i < kLearnCoarse; i += threads) s_cb[i] = tb.fb[i * kLearnFine];
   }
-  __device__ __forceinline__ uint32_t operator()(uint32_t key) const { return coarse_of(s_cb, key); }
+  __device__ __forceinline__ uint32_t operator()(uint32_t key) const {
+    uint32_t lo, hi;
+    cell_bounds(as_lds<const uint16_t>(s_ctab), s_kmin, s_cshift, key, lo, hi);
+    return search_range(as_lds<const uint32_t>(s_cb), 0u, lo >> 6, hi >> 6, key);
+  }
 };
 struct FineDigit {
-  const uint32_t* fb;
+  LearnTables tb;
   __device__ __forceinline__ void setup(int32_t seg, int tid, int threads) const {
-    for (int i = tid; i < kLearnFine; i += threads) s_fbs[i] = fb[seg * kLearnFine + i];
+    load_cells(tb, tid, threads);
+    for (int i = tid; i < kLearnFine; i += threads) s_fbs[i] = tb.fb[seg * kLearnFine + i];
+    if (tid == 0) s_seg_id = static_cast<uint32_t>(seg);
   }
-  __device__ __forceinline__ uint32_t operator()(uint32_t key) const { return fine_of(s_fbs, key); }
+  __device__ __forceinline__ uint32_t operator()(uint32_t key) const {
+    uint32_t lo, hi;
+    cell_bounds(as_lds<const uint16_t>(s_ctab), s_kmin, s_cshift, key, lo, hi);
+    const uint32_t base = s_seg_id * kLearnFine;  // every key of the tile is in this segment
+    lo = max(lo, base);
+    hi = min(hi, base + kLearnFine - 1);
+    return search_range(as_lds<const uint32_t>(s_fbs), base, lo, hi, key) - base;
+  }
 };
 
 // ------------------------------------------------------------ histogram
 constexpr int kHistThreads = 1024;
 
 __global__ __launch_bounds__(kHistThreads) void lhist_kernel(const uint32_t* __restrict__ keys,
-                                                             int64_t n,
-                                                             const uint32_t* __restrict__ fb,
+                                                             int64_t n, LearnTables tb,
                                                              uint32_t* __restrict__ rows) {
   __shared__ uint32_t s_fb[kLearnBuckets];
   __shared__ uint32_t s_cnt[kLearnBuckets];
-  __shared__ uint32_t s_c[kLearnCoarse];
   const int tid = threadIdx.x;
   for (int i = tid; i < kLearnBuckets; i += kHistThreads) {
-    s_fb[i] = fb[i];
+    s_fb[i] = tb.fb[i];
     s_cnt[i] = 0;
   }
-  for (int i = tid; i < kLearnCoarse; i += kHistThreads) s_c[i] = fb[i * kLearnFine];
+  load_cells(tb, tid, kHistThreads);
   __syncthreads();
+  const auto ctab = as_lds<const uint16_t>(s_ctab);
+  const auto fb = as_lds<const uint32_t>(s_fb);
+  const uint32_t kmin = s_kmin, cshift = s_cshift;
+  auto count = [&](uint32_t key) {
+    uint32_t lo, hi;
+    cell_bounds(ctab, kmin, cshift, key, lo, hi);
+    atomicAdd(&s_cnt[search_range(fb, 0u, lo, hi, key)], 1u);
+  };
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kHistThreads;
   int64_t i = static_cast<int64_t>(blockIdx.x) * kHistThreads + tid;
   constexpr int U = 16;
@@ -152,18 +203,9 @@ __global__ __launch_bounds__(kHistThreads) void lhist_kernel(const uint32_t* __r
 #pragma unroll
     for (int u = 0; u < U; ++u) k[u] = keys[i + u * stride];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t c = coarse_of(s_c, k[u]);
-      const uint32_t f = fine_of(s_fb + c * kLearnFine, k[u]);
-      atomicAdd(&s_cnt[c * kLearnFine + f], 1u);
-    }
+    for (int u = 0; u < U; ++u) count(k[u]);
   }
-  for (; i < n; i += stride) {
-    const uint32_t key = keys[i];
-    const uint32_t c = coarse_of(s_c, key);
-    const uint32_t f = fine_of(s_fb + c * kLearnFine, key);
-    atomicAdd(&s_cnt[c * kLearnFine + f], 1u);
-  }
+  for (; i < n; i += stride) count(keys[i]);
   __syncthreads();
   uint32_t* row = rows + static_cast<int64_t>(blockIdx.x) * kLearnBuckets;
   for (int q = tid; q < kLearnBuckets; q += kHistThreads) row[q] = s_cnt[q];
@@ -652,6 +694,25 @@ __global__ __launch_bounds__(1024) void lmonotone_kernel(uint32_t* __restrict__ 
   }
 }
 
+// Cell table of the compiled boundaries: cshift so that 4096 cells of
+// 2^cshift keys cover [kmin, kmax], then the bucket of every cell's first
+// key (2^32 - 1 past the end of the key space).
+__global__ __launch_bounds__(256) void lcells_kernel(const uint32_t* __restrict__ fb,
+                                                     uint32_t* __restrict__ meta,
+                                                     uint16_t* __restrict__ ctab) {
+  const int32_t c = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t kmin = meta[0], span = meta[1] - meta[0];
+  const uint32_t bits = span ? 32u - static_cast<uint32_t>(__clz(static_cast<int>(span))) : 0u;
+  const uint32_t cshift = bits > 12u ? bits - 12u : 0u;
+  if (c == 0) meta[2] = cshift;
+  if (c > kLearnCells) return;
+  const uint64_t k64 = static_cast<uint64_t>(kmin) + (static_cast<uint64_t>(c) << cshift);
+  const uint32_t key = k64 > 0xffffffffull ? 0xffffffffu : static_cast<uint32_t>(k64);
+  uint32_t f = 0;
+  for (uint32_t step = kLearnBuckets / 2; step > 0; step >>= 1) f += fb[f + step] <= key ? step : 0u;
+  ctab[c] = static_cast<uint16_t>(f);
+}
+
 // ------------------------------------------------------------ diagnostics
 // Mean gradient of 0.5 * (r(x) - t)^2 over n pairs, accumulated into grad
 // (zeroed by the caller).
@@ -685,13 +746,18 @@ __global__ __launch_bounds__(256) void lbucket_kernel(const uint32_t* __restrict
                                                       int32_t n, const float* __restrict__ model,
                                                       const uint32_t* __restrict__ meta,
                                                       const uint32_t* __restrict__ kleaf,
-                                                      const uint32_t* __restrict__ fb, int mode,
+                                                      const uint32_t* __restrict__ fb,
+                                                      const uint16_t* __restrict__ ctab, int mode,
                                                       uint32_t* __restrict__ q) {
   const int32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const uint32_t key = keys[i];
   if (mode == 0) {
     q[i] = q_model(cdf_view(model, meta, kleaf), key);
+  } else if (mode == 2) {
+    uint32_t lo, hi;
+    cell_bounds(ctab, meta[0], meta[2], key, lo, hi);
+    q[i] = search_range(fb, 0u, lo, hi, key);
   } else {
     uint32_t f = 0;
     for (uint32_t step = kLearnBuckets / 2; step > 0; step >>= 1)
@@ -708,7 +774,7 @@ void launch_lhist(const uint32_t* keys, int64_t n, const LearnTables& tb, int32_
   const int64_t want = (n + 65535) / 65536;
   const int32_t g = static_cast<int32_t>(std::max<int64_t>(1, std::min<int64_t>(want, kLearnHistGrid)));
   *n_rows = g;
-  lhist_kernel<<<g, kHistThreads, 0, s>>>(keys, n, tb.fb, tb.rows);
+  lhist_kernel<<<g, kHistThreads, 0, s>>>(keys, n, tb, tb.rows);
   lreduce_kernel<<<kLearnBuckets / 256, 256, 0, s>>>(tb.rows, g, tb.hist);
 }
 
@@ -725,7 +791,7 @@ void launch_lpass_a(const uint32_t* in, uint32_t* out, int32_t n, const LearnTab
   const int32_t tiles = (n + kLearnPassTile - 1) / kLearnPassTile;
   onesweep_kernel<kLearnPassWaves, kLearnPassKpt, CoarseDigit>
       <<<tiles, kLearnPassWaves * 64, 0, s>>>(in, out, sv, nullptr, nullptr, tb.hist_a, 0, flags,
-                                               ctrl, kStageBucket, CoarseDigit{tb.fb});
+                                               ctrl, kStageBucket, CoarseDigit{tb});
 }
 
 int32_t learn_pass_b_max_tiles(int32_t n) {
@@ -742,7 +808,7 @@ void launch_lpass_b(const uint32_t* in, uint32_t* out, int32_t n, const LearnTab
   onesweep_kernel<kLearnPassWaves, kLearnPassKpt, FineDigit>
       <<<learn_pass_b_max_tiles(n), kLearnPassWaves * 64, 0, s>>>(
           in, out, sv, tb.tile_start, tb.tiles_b, tb.hist_b, 0, flags, ctrl, kStageBucket + 1,
-          FineDigit{tb.fb});
+          FineDigit{tb});
 }
 
 void launch_lsort(uint32_t* keys, const LearnTables& tb, hipStream_t s) {
@@ -784,10 +850,11 @@ void launch_lfit_leaves(const uint32_t* sorted, int32_t ns, const uint32_t* klea
   lfit_leaves_kernel<<<kCdfLeaves / 256, 256, 0, s>>>(sorted, ns, kleaf, model);
 }
 
-void launch_lboundaries(const float* model, const uint32_t* meta, const uint32_t* kleaf,
-                        uint32_t* fb, hipStream_t s) {
+void launch_lboundaries(const float* model, uint32_t* meta, const uint32_t* kleaf,
+                        uint32_t* fb, uint16_t* ctab, hipStream_t s) {
   lboundaries_kernel<<<kLearnBuckets / 256, 256, 0, s>>>(model, meta, kleaf, fb);
   lmonotone_kernel<<<1, 1024, 0, s>>>(fb);
+  lcells_kernel<<<(kLearnCells + 1 + 255) / 256, 256, 0, s>>>(fb, meta, ctab);
 }
 
 void launch_llinear(float* model, hipStream_t s) { llinear_root_kernel<<<1, 64, 0, s>>>(model); }
@@ -803,9 +870,10 @@ void launch_lgrad(const float* xs, const float* ts, int32_t n, const float* mode
 }
 
 void launch_lbucket(const uint32_t* keys, int32_t n, const float* model, const uint32_t* meta,
-                    const uint32_t* kleaf, const uint32_t* fb, int mode, uint32_t* q,
-                    hipStream_t s) {
-  if (n > 0) lbucket_kernel<<<(n + 255) / 256, 256, 0, s>>>(keys, n, model, meta, kleaf, fb, mode, q);
+                    const uint32_t* kleaf, const uint32_t* fb, const uint16_t* ctab, int mode,
+                    uint32_t* q, hipStream_t s) {
+  if (n > 0)
+    lbucket_kernel<<<(n + 255) / 256, 256, 0, s>>>(keys, n, model, meta, kleaf, fb, ctab, mode, q);
 }
 
 }  // namespace ocp

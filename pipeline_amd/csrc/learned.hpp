@@ -30,9 +30,16 @@ constexpr int kLearnBigCap = 16384;   // larger buckets go to the OneSweep fallb
 constexpr int kLearnPassWaves = 8, kLearnPassKpt = 12;
 constexpr int kLearnPassTile = kLearnPassWaves * 64 * kLearnPassKpt;
 constexpr int kLearnHistGrid = 256;   // histogram workgroups (rows)
+// Cell table: the key range [kmin, kmin + 4096 << cshift) in 4096 equal
+// cells; ctab[c] = bucket of the cell's first key, so a key's bucket lies in
+// [ctab[c], ctab[c+1]] and most keys need no search at all.
+constexpr int kLearnCells = 4096;
+// meta words on device: [0] kmin, [1] kmax, [2] cshift
 
 struct LearnTables {
   const uint32_t* fb;   // [16384] bucket boundaries: q(key) >= f  <=>  key >= fb[f]
+  const uint16_t* ctab; // [4097] bucket of each cell's first key
+  const uint32_t* meta; // kmin, kmax, cshift
   uint32_t* rows;       // [kLearnHistGrid][16384] per-workgroup bucket counts
   uint32_t* hist;       // [16384]
   int32_t* bstart;      // [16385] bucket starts
@@ -81,8 +88,9 @@ void launch_lleaf_starts(const float* model, const uint32_t* meta, uint32_t* kle
                          hipStream_t s);
 void launch_lfit_leaves(const uint32_t* sorted, int32_t ns, const uint32_t* kleaf,
                         float* model, hipStream_t s);
-void launch_lboundaries(const float* model, const uint32_t* meta, const uint32_t* kleaf,
-                        uint32_t* fb, hipStream_t s);
+// fb, then cshift (meta[2]) and the cell table
+void launch_lboundaries(const float* model, uint32_t* meta, const uint32_t* kleaf,
+                        uint32_t* fb, uint16_t* ctab, hipStream_t s);
 // identity root over [meta[0], meta[1]], uniform leaves (the linear CDF)
 void launch_llinear(float* model, hipStream_t s);
 void launch_llinear_leaves(const uint32_t* kleaf, const uint32_t* meta, float* model,
@@ -91,9 +99,10 @@ void launch_llinear_leaves(const uint32_t* kleaf, const uint32_t* meta, float* m
 // ---- diagnostics
 void launch_lgrad(const float* xs, const float* ts, int32_t n, const float* model,
                   float* grad, hipStream_t s);
-// q from the model itself (mode 0) or from the compiled boundaries (mode 1)
+// q from the model itself (mode 0), from a binary search of the compiled
+// boundaries (mode 1) or through the cell table (mode 2, the sort's way)
 void launch_lbucket(const uint32_t* keys, int32_t n, const float* model, const uint32_t* meta,
-                    const uint32_t* kleaf, const uint32_t* fb, int mode, uint32_t* q,
-                    hipStream_t s);
+                    const uint32_t* kleaf, const uint32_t* fb, const uint16_t* ctab, int mode,
+                    uint32_t* q, hipStream_t s);
 
 }  // namespace ocp

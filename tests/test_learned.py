@@ -118,8 +118,10 @@ def test_gpu_training_and_compiled_boundaries(ctx):
     d = ctx.upload(probe)
     q0 = ctx.cdf_buckets(d, mode=0)
     q1 = ctx.cdf_buckets(d, mode=1)
+    q2 = ctx.cdf_buckets(d, mode=2)  # through the cell table, as the sort does
     d.free()
     np.testing.assert_array_equal(q0, q1)
+    np.testing.assert_array_equal(q0, q2)
     kleaf = cdf.leaf_starts(p, kmin, kmax)
     np.testing.assert_array_equal(q0, cdf.q_model(p, kmin, kmax, kleaf, probe))
 
