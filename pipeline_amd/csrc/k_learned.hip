@@ -131,7 +131,7 @@ __device__ __forceinline__ uint32_t search_range(P32 tab, uint32_t base, uint32_
 __shared__ uint16_t s_ctab[kLearnCells + 2];
 __shared__ uint32_t s_cb[kLearnCoarse];
 __shared__ uint32_t s_fbs[kLearnFine];
-__shared__ uint32_t s_kmin, s_cshift, s_seg_id;
+__shared__ uint32_t s_kmin, s_cshift;
 
 __device__ __forceinline__ void load_cells(const LearnTables& tb, int tid, int threads) {
   const uint32_t* src = reinterpret_cast<const uint32_t*>(tb.ctab);
@@ -143,32 +143,35 @@ __device__ __forceinline__ void load_cells(const LearnTables& tb, int tid, int t
   }
 }
 
+// Per-key branch-free binary search over the 256 coarse boundaries (pass A)
+// or the 64 boundaries of the tile's coarse segment (pass B). In the
+// scatter passes this measured faster than the cell table, whose 8 KB must
+// be loaded by every tile.
 struct CoarseDigit {
   LearnTables tb;
   __device__ __forceinline__ void setup(int32_t, int tid, int threads) const {
-    load_cells(tb, tid, threads);
     for (int i = tid; i < kLearnCoarse; i += threads) s_cb[i] = tb.fb[i * kLearnFine];
   }
   __device__ __forceinline__ uint32_t operator()(uint32_t key) const {
-    uint32_t lo, hi;
-    cell_bounds(as_lds<const uint16_t>(s_ctab), s_kmin, s_cshift, key, lo, hi);
-    return search_range(as_lds<const uint32_t>(s_cb), 0u, lo >> 6, hi >> 6, key);
+    const auto cb = as_lds<const uint32_t>(s_cb);
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t step = kLearnCoarse / 2; step > 0; step >>= 1) c += cb[c + step] <= key ? step : 0u;
+    return c;
   }
 };
 struct FineDigit {
   LearnTables tb;
   __device__ __forceinline__ void setup(int32_t seg, int tid, int threads) const {
-    load_cells(tb, tid, threads);
     for (int i = tid; i < kLearnFine; i += threads) s_fbs[i] = tb.fb[seg * kLearnFine + i];
-    if (tid == 0) s_seg_id = static_cast<uint32_t>(seg);
   }
   __device__ __forceinline__ uint32_t operator()(uint32_t key) const {
-    uint32_t lo, hi;
-    cell_bounds(as_lds<const uint16_t>(s_ctab), s_kmin, s_cshift, key, lo, hi);
-    const uint32_t base = s_seg_id * kLearnFine;  // every key of the tile is in this segment
-    lo = max(lo, base);
-    hi = min(hi, base + kLearnFine - 1);
-    return search_range(as_lds<const uint32_t>(s_fbs), base, lo, hi, key) - base;
+    // every key of the tile is in this coarse segment: fbs[0] <= key
+    const auto fbs = as_lds<const uint32_t>(s_fbs);
+    uint32_t f = 0;
+#pragma unroll
+    for (uint32_t step = kLearnFine / 2; step > 0; step >>= 1) f += fbs[f + step] <= key ? step : 0u;
+    return f;
   }
 };
 
@@ -364,7 +367,36 @@ __global__ __launch_bounds__(WAVES * 64) void lsort_kernel(uint32_t* __restrict_
   }
   const uint64_t lt = lanemask_lt();
   uint32_t* wh = s_whist + w * 256;
-  for (int pass = 0; pass < passes; ++pass) {
+  // Pass 0 needs no stability (no earlier order to keep), so it is a plain
+  // counting sort with LDS atomics; padding stays out of it and is parked at
+  // the top of the array, where the stable passes keep it last.
+  {
+    for (int i = tid; i < 256; i += THREADS) s_whist[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
+      if (idx < n) atomicAdd(&s_whist[kv[k] & 255u], 1u);
+    }
+    __syncthreads();
+    const uint32_t cnt = tid < 256 ? s_whist[tid] : 0u;
+    uint32_t all;
+    const uint32_t ds = block_excl_scan<THREADS>(cnt, s_tmp, &all);
+    if (tid < 256) s_whist[tid] = ds;  // cursor per digit
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
+      if (idx < n) s_keys[atomicAdd(&s_whist[kv[k] & 255u], 1u)] = kv[k];
+    }
+    for (int i = n + tid; i < CAP; i += THREADS) s_keys[i] = 0xffffffffu;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KPT; ++k)
+      kv[k] = s_keys[static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane)];
+    __syncthreads();
+  }
+  for (int pass = 1; pass < passes; ++pass) {
     const uint32_t shift = 8u * static_cast<uint32_t>(pass);
     for (int i = tid; i < WAVES * 256; i += THREADS) s_whist[i] = 0;
     __syncthreads();

@@ -27,7 +27,7 @@ constexpr int kLearnFine = 64;        // second scatter digit q & 63
 constexpr int kLearnSmallCap = 4096;  // bucket-sort classes (keys per bucket)
 constexpr int kLearnMidCap = 8192;
 constexpr int kLearnBigCap = 16384;   // larger buckets go to the OneSweep fallback
-constexpr int kLearnPassWaves = 8, kLearnPassKpt = 12;
+constexpr int kLearnPassWaves = 8, kLearnPassKpt = 16;
 constexpr int kLearnPassTile = kLearnPassWaves * 64 * kLearnPassKpt;
 constexpr int kLearnHistGrid = 256;   // histogram workgroups (rows)
 // Cell table: the key range [kmin, kmin + 4096 << cshift) in 4096 equal
