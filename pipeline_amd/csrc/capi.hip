@@ -101,7 +101,6 @@ struct LearnState {
   // per-sort tables
   int64_t n_cap = 0;
   LearnTables tb{};
-  uint32_t* big_hist = nullptr;  // big_cap * 1024
   int32_t* h_info = nullptr;     // pinned host copy of tb.info
   hipEvent_t ev_info = nullptr;
 };
@@ -556,7 +555,7 @@ void free_learn(LearnState& L) {
   dev_free(t.rows); dev_free(t.hist); dev_free(t.bstart); dev_free(t.hist_a);
   dev_free(t.hist_b); dev_free(t.seg_start); dev_free(t.tile_start); dev_free(t.tiles_b);
   dev_free(t.big_start); dev_free(t.big_end); dev_free(t.big_tiles); dev_free(t.info);
-  dev_free(L.big_hist);
+  dev_free(t.big_hist);
   if (L.h_info) (void)hipHostFree(L.h_info);
   if (L.ev_info) (void)hipEventDestroy(L.ev_info);
   L = LearnState{};
@@ -1115,7 +1114,7 @@ static int learn_tables_alloc(ocp_ctx c, int64_t n) {
   if (!t.rows) {
     int r = 0;
     if ((r = dev_alloc(c, &t.rows, int64_t(kLearnHistGrid) * kLearnBuckets)) ||
-        (r = dev_alloc(c, &t.hist, kLearnBuckets)) ||
+        (r = dev_alloc(c, &t.hist, int64_t(kLearnReduceGroups) * kLearnBuckets)) ||
         (r = dev_alloc(c, &t.bstart, kLearnBuckets + 1)) || (r = dev_alloc(c, &t.hist_a, 1024)) ||
         (r = dev_alloc(c, &t.hist_b, int64_t(kLearnCoarse) * 1024)) ||
         (r = dev_alloc(c, &t.seg_start, kLearnCoarse + 1)) ||
@@ -1133,11 +1132,11 @@ static int learn_tables_alloc(ocp_ctx c, int64_t n) {
   if (n > L.n_cap) {
     const int64_t cap = std::max<int64_t>(n, 1 << 20);
     const int32_t big_cap = static_cast<int32_t>(cap / (kLearnBigCap + 1) + 1);
-    dev_free(t.big_start); dev_free(t.big_end); dev_free(t.big_tiles); dev_free(L.big_hist);
+    dev_free(t.big_start); dev_free(t.big_end); dev_free(t.big_tiles); dev_free(t.big_hist);
     int r = 0;
     if ((r = dev_alloc(c, &t.big_start, big_cap)) || (r = dev_alloc(c, &t.big_end, big_cap)) ||
         (r = dev_alloc(c, &t.big_tiles, int64_t(big_cap) + 1)) ||
-        (r = dev_alloc(c, &L.big_hist, int64_t(big_cap) * 1024)))
+        (r = dev_alloc(c, &t.big_hist, int64_t(big_cap) * 1024)))
       return r;
     t.big_cap = big_cap;
     L.n_cap = cap;
@@ -1387,19 +1386,19 @@ int ocp_sort_learned(ocp_ctx c, uint32_t* keys, uint32_t* alt, uint32_t n,
   if (n_big > tb.big_cap) return fail(c, OCP_ERR_STATE, "oversized-bucket table overflow");
   if (n_big > 0) {
     // Oversized buckets: segmented OneSweep, in place (4 passes end in keys).
-    launch_fill_u32(L.big_hist, 0u, int64_t(n_big) * 1024, c->stream);
+    launch_fill_u32(tb.big_hist, 0u, int64_t(n_big) * 1024, c->stream);
     SegView sv{};
     sv.n_seg = n_big;
     sv.start = tb.big_start;
     sv.end = tb.big_end;
     sv.n1 = ni;
-    launch_histogram_tiles(keys, sv, tb.big_tiles, big_tiles, stile, L.big_hist, c->tune,
+    launch_histogram_tiles(keys, sv, tb.big_tiles, big_tiles, stile, tb.big_hist, c->tune,
                            c->stream);
     uint32_t* src = keys;
     uint32_t* dst = alt;
     for (int pass = 0; pass < 4; ++pass) {
-      launch_onesweep_pass(src, dst, sv, tb.big_tiles, big_tiles, L.big_hist, c->s_flags, c->ctrl,
-                           pass, c->tune, c->stream);
+      launch_onesweep_pass(src, dst, sv, tb.big_tiles, big_tiles, tb.big_hist, c->s_flags,
+                           c->ctrl, pass, c->tune, c->stream);
       std::swap(src, dst);
     }
   }

@@ -214,14 +214,33 @@ __global__ __launch_bounds__(kHistThreads) void lhist_kernel(const uint32_t* __r
   for (int q = tid; q < kLearnBuckets; q += kHistThreads) row[q] = s_cnt[q];
 }
 
+// Column sums of the matrix in kLearnReduceGroups groups of rows (enough
+// workgroups to keep many loads in flight), then the group partials are
+// added into the first row.
 __global__ __launch_bounds__(256) void lreduce_kernel(const uint32_t* __restrict__ rows,
                                                       int32_t n_rows,
-                                                      uint32_t* __restrict__ hist) {
+                                                      uint32_t* __restrict__ partial) {
   const int32_t q = blockIdx.x * 256 + threadIdx.x;
-  if (q >= kLearnBuckets) return;
-  uint32_t s = 0;
-  for (int32_t r = 0; r < n_rows; ++r) s += rows[static_cast<int64_t>(r) * kLearnBuckets + q];
-  hist[q] = s;
+  const int32_t per = (n_rows + kLearnReduceGroups - 1) / kLearnReduceGroups;
+  const int32_t r0 = blockIdx.y * per;
+  const int32_t r1 = min(n_rows, r0 + per);
+  uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int32_t r = r0;
+  for (; r + 8 <= r1; r += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] += rows[static_cast<int64_t>(r + u) * kLearnBuckets + q];
+  }
+  for (; r < r1; ++r) acc[0] += rows[static_cast<int64_t>(r) * kLearnBuckets + q];
+  partial[static_cast<int64_t>(blockIdx.y) * kLearnBuckets + q] =
+      acc[0] + acc[1] + acc[2] + acc[3] + acc[4] + acc[5] + acc[6] + acc[7];
+}
+
+__global__ __launch_bounds__(256) void lreduce2_kernel(uint32_t* __restrict__ hist) {
+  const int32_t q = blockIdx.x * 256 + threadIdx.x;
+  uint32_t v = 0;
+#pragma unroll
+  for (int gi = 0; gi < kLearnReduceGroups; ++gi) v += hist[gi * kLearnBuckets + q];
+  hist[q] = v;
 }
 
 // ------------------------------------------------------------ tables
@@ -281,8 +300,11 @@ __global__ __launch_bounds__(1024) void lprefix_kernel(LearnTables tb, int32_t f
   if (tid == 0) tb.big_tiles[min(static_cast<int32_t>(n_big_all), tb.big_cap)] =
       static_cast<int32_t>(n_tiles_all);
   // coarse segments: counts, starts, tile map; fine counts per segment
-  for (int i = tid; i < kLearnBuckets; i += 1024)
-    tb.hist_b[(i / kLearnFine) * 1024 + (i % kLearnFine)] = tb.hist[i];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int i = tid * PER + q;
+    tb.hist_b[(i / kLearnFine) * 1024 + (i % kLearnFine)] = v[q];
+  }
   uint32_t cnt = 0;
   if (tid < kLearnCoarse) {
     cnt = static_cast<uint32_t>(s_bs[(tid + 1) * kLearnFine] - s_bs[tid * kLearnFine]);
@@ -807,7 +829,8 @@ void launch_lhist(const uint32_t* keys, int64_t n, const LearnTables& tb, int32_
   const int32_t g = static_cast<int32_t>(std::max<int64_t>(1, std::min<int64_t>(want, kLearnHistGrid)));
   *n_rows = g;
   lhist_kernel<<<g, kHistThreads, 0, s>>>(keys, n, tb, tb.rows);
-  lreduce_kernel<<<kLearnBuckets / 256, 256, 0, s>>>(tb.rows, g, tb.hist);
+  lreduce_kernel<<<dim3(kLearnBuckets / 256, kLearnReduceGroups), 256, 0, s>>>(tb.rows, g, tb.hist);
+  lreduce2_kernel<<<kLearnBuckets / 256, 256, 0, s>>>(tb.hist);
 }
 
 void launch_lprefix(const LearnTables& tb, int32_t, int32_t fallback_tile, hipStream_t s) {

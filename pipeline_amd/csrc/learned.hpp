@@ -30,6 +30,7 @@ constexpr int kLearnBigCap = 16384;   // larger buckets go to the OneSweep fallb
 constexpr int kLearnPassWaves = 8, kLearnPassKpt = 16;
 constexpr int kLearnPassTile = kLearnPassWaves * 64 * kLearnPassKpt;
 constexpr int kLearnHistGrid = 256;   // histogram workgroups (rows)
+constexpr int kLearnReduceGroups = 8; // row groups of the column sums
 // Cell table: the key range [kmin, kmin + 4096 << cshift) in 4096 equal
 // cells; ctab[c] = bucket of the cell's first key, so a key's bucket lies in
 // [ctab[c], ctab[c+1]] and most keys need no search at all.
@@ -41,7 +42,7 @@ struct LearnTables {
   const uint16_t* ctab; // [4097] bucket of each cell's first key
   const uint32_t* meta; // kmin, kmax, cshift
   uint32_t* rows;       // [kLearnHistGrid][16384] per-workgroup bucket counts
-  uint32_t* hist;       // [16384]
+  uint32_t* hist;       // [kLearnReduceGroups][16384] partial sums; row 0 ends as the total
   int32_t* bstart;      // [16385] bucket starts
   uint32_t* hist_a;     // [1024] first scatter's digit counts (256 used)
   uint32_t* hist_b;     // [256 * 1024] second scatter's counts per coarse segment (64 used)
@@ -52,6 +53,7 @@ struct LearnTables {
   int32_t* big_end;     // [big_cap]
   int32_t* big_tiles;   // [big_cap + 1] their tile map (fallback sort tile)
   int32_t* info;        // [8] see LearnInfo
+  uint32_t* big_hist;   // [big_cap * 1024] digit histograms of the oversized buckets
   int32_t big_cap;
 };
 enum LearnInfo : int {
