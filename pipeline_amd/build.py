@@ -52,9 +52,30 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_c_demo()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+ROOT = os.path.dirname(HERE)
+C_DEMO_SRC = os.path.join(ROOT, "tests", "c", "abi_demo.c")
+C_DEMO = os.path.join(OUT_DIR, "abi_demo")
+
+
+def build_c_demo(out: str = C_DEMO) -> str:
+    """Compiles the plain-C caller of include/octree_pipeline.h (C99, no C++)
+    and links it against the library: the header is valid C and every entry
+    point it uses links."""
+    cc = os.environ.get("CC", shutil.which("gcc") or "cc")
+    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    cmd = [cc, "-std=c99", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+           C_DEMO_SRC, "-o", out, "-L", OUT_DIR, "-locpipe", "-L", rocm_lib,
+           f"-Wl,-rpath,{OUT_DIR}", f"-Wl,-rpath,{rocm_lib}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C demo build failed:\n{r.stdout}\n{r.stderr}")
+    return out
 
 
 if __name__ == "__main__":

@@ -77,3 +77,31 @@ def test_generators_are_seeded_and_clipped():
         q = (seg - mn[k]) / rg[k] * 1024
         assert q.min() >= 0 and q.max() < 1024
     assert pipeline_amd.check_batch(offs, mn, rg) == 0
+
+
+def _lcg_points(n=4096, seed=12345):
+    """The points tests/c/abi_demo.c generates (32-bit LCG, float32 ops)."""
+    out = np.zeros((n, 4), np.float32)
+    flat = out.reshape(-1)
+    st = seed
+    for i in range(n * 4):
+        st = (st * 1664525 + 1013904223) & 0xFFFFFFFF
+        if i % 4 != 3:
+            flat[i] = np.float32(st >> 8) / np.float32(16777216.0) * np.float32(1000.0)
+    return out
+
+
+def test_c_caller_compiles_and_links(tmp_path):
+    from pipeline_amd import build as B
+    exe = B.build_c_demo(str(tmp_path / "abi_demo"))
+    assert os.path.exists(exe)
+
+
+@pytest.mark.gpu
+def test_c_caller_runs():
+    from pipeline_amd import build as B
+    exp = oracle.build_plain(_lcg_points())
+    assert os.path.exists(B.C_DEMO), "build the library first (pipeline_amd/build.py)"
+    r = subprocess.run([B.C_DEMO, str(exp["n_unique"]), str(exp["n_oct"])],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
