@@ -41,8 +41,12 @@
 // chains and most child lookups of the group walk are LDS reads (anything
 // outside the window falls back to global memory); the backward walk stops
 // as soon as no earlier chain can reach this one, using the largest edge
-// count of the window. Nodes are assembled in registers and written with
-// three 16-byte and one 12-byte store.
+// count of the window. The group walk is a depth-first loop per lane, so a
+// wave runs as many steps as its largest group rather than a fixed tree of
+// visits. Each chain keeps the order key of the last writer per child index
+// of its first slot in an LDS column and decodes the values when it writes
+// the node. Nodes are assembled in registers and written with three 16-byte
+// and one 12-byte store.
 //
 // Edge counts are at most 10 (prefix lengths are at most 31 for 30-bit
 // codes), so a chain spans at most 10 slots.
@@ -97,6 +101,7 @@ struct View {
   int32_t nc;
   lds_ptr<const v4i> rec;
   lds_ptr<const uint32_t> code;
+  lds_ptr<uint32_t> sc_col;  // this thread's SlotChildren column
   __device__ __forceinline__ Rec R(int32_t q) const {
     if (static_cast<uint32_t>(q - lo) < static_cast<uint32_t>(n)) {
       const v4i r = rec[q - lo];
@@ -125,28 +130,27 @@ __device__ __forceinline__ uint32_t delta(uint32_t a, uint32_t b) {
   return ((x ? static_cast<uint32_t>(__clz(static_cast<int>(x))) : 32u) - 1u) & 0xffu;
 }
 
-// Per-child-index last writer of one slot, kept in registers.
+// Per-child-index last writer of one slot, as its order key only: a make
+// write (chain q, step t) always stores P[q] + t and a link write (node x,
+// side) stores left_child[x] + side, so the value follows from the key.
+// Writers have distinct keys and the largest key is the last writer; key 0
+// is the untouched state (value 0). One LDS word per child index in the
+// thread's own column, kept by an LDS atomic max: a register table needs a
+// compare-and-select over all eight entries per write.
 struct SlotChildren {
-  uint32_t key[8];
-  int32_t val[8];
+  lds_ptr<uint32_t> col;  // entry c at col[c * kThreads]
   uint32_t node_mask;
-  __device__ __forceinline__ void init() {
+  __device__ __forceinline__ void init(lds_ptr<uint32_t> column) {
+    col = column;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      key[c] = 0;
-      val[c] = 0;
-    }
+    for (int c = 0; c < 8; ++c) col[c * kThreads] = 0;
     node_mask = 0;
   }
-  __device__ __forceinline__ void put(uint32_t which, uint32_t k, int32_t v) {
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      if (static_cast<uint32_t>(c) == which && k > key[c]) {
-        key[c] = k;
-        val[c] = v;
-      }
-    }
+  __device__ __forceinline__ void put(uint32_t which, uint32_t k) {
+    __hip_atomic_fetch_max(col + which * kThreads, k, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
   }
+  __device__ __forceinline__ uint32_t key(int c) const { return col[c * kThreads]; }
 };
 
 __device__ __forceinline__ uint32_t make_key(int32_t chain, int32_t step) {
@@ -154,6 +158,15 @@ __device__ __forceinline__ uint32_t make_key(int32_t chain, int32_t step) {
 }
 __device__ __forceinline__ uint32_t link_key(int32_t node, int side) {
   return 0x80000000u | static_cast<uint32_t>((node << 1) | side);
+}
+// The child value a write with this key stores.
+__device__ __forceinline__ int32_t key_value(const View& v, uint32_t k) {
+  if (k == 0) return 0;
+  if (k & 0x80000000u) {
+    const int32_t x = static_cast<int32_t>((k & 0x7fffffffu) >> 1);
+    return v.R(x).child() + static_cast<int32_t>(k & 1u);
+  }
+  return v.P(static_cast<int32_t>((k - 1u) >> 4)) + static_cast<int32_t>((k - 1u) & 15u);
 }
 
 // Child `side` of group node x (record rx): a leaf link, a top link of a
@@ -163,7 +176,7 @@ __device__ __forceinline__ int32_t visit_child(const View& v, int32_t x, const R
   const int32_t ch = rx.child() + side;
   if (rx.leaf(side)) {
     if (ch < 0 || ch > v.b.m) return -1;  // leaves index the m+1 codes
-    sc.put(child_of(v.CODE(ch), rx.pn / 3 + 1), link_key(x, side), ch);
+    sc.put(child_of(v.CODE(ch), rx.pn / 3 + 1), link_key(x, side));
     return -1;
   }
   if (ch <= 0 || ch >= v.b.m) return -1;
@@ -174,43 +187,44 @@ __device__ __forceinline__ int32_t visit_child(const View& v, int32_t x, const R
   }
   if (ry.e > 0) {
     const uint32_t w = child_of(v.CODE(ch), ry.pn / 3 - ry.e + 1);
-    sc.put(w, make_key(ch, ry.e - 1), ry.p + ry.e - 1);
+    sc.put(w, make_key(ch, ry.e - 1));
     sc.node_mask |= 1u << w;
   }
   return -1;
 }
 
-// Members below the given ones, down to two further levels.
-__device__ __forceinline__ void visit_members(const View& v, int32_t m0, const Rec& r0,
-                                              int32_t m1, const Rec& r1, SlotChildren& sc) {
-  int32_t l2[4] = {-1, -1, -1, -1};
-  Rec q2[4];
-  if (m0 >= 0) {
-    l2[0] = visit_child(v, m0, r0, 0, sc, &q2[0]);
-    l2[1] = visit_child(v, m0, r0, 1, sc, &q2[1]);
-  }
-  if (m1 >= 0) {
-    l2[2] = visit_child(v, m1, r1, 0, sc, &q2[2]);
-    l2[3] = visit_child(v, m1, r1, 1, sc, &q2[3]);
-  }
+// Depth-first walk of a group from node x at depth d: both child visits of
+// every member down to depth dmax (e == 0 children below that are not
+// followed). Four pending members suffice: each level below x leaves at
+// most one sibling behind, plus the two children of the deepest level.
+__device__ __forceinline__ void walk_group(const View& v, int32_t x, Rec rx, int d, int dmax,
+                                           SlotChildren& sc) {
+  int32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;  // pending members, s0 on top
+  uint32_t ds = 0;                         // their depths, 4 bits each, top lowest
+  int n = 0;
+  while (true) {
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    if (l2[q] < 0) continue;
-    Rec g[2];
-    const int32_t g0 = visit_child(v, l2[q], q2[q], 0, sc, &g[0]);
-    const int32_t g1 = visit_child(v, l2[q], q2[q], 1, sc, &g[1]);
-    // Below that every child is a leaf or starts a chain: prefix lengths
-    // would otherwise exceed the three values of one octree level.
-    if (g0 >= 0) {
-      Rec dummy;
-      (void)visit_child(v, g0, g[0], 0, sc, &dummy);
-      (void)visit_child(v, g0, g[0], 1, sc, &dummy);
+    for (int side = 0; side < 2; ++side) {
+      Rec ry;
+      const int32_t y = visit_child(v, x, rx, side, sc, &ry);
+      if (y >= 0 && d < dmax) {
+        s3 = s2;
+        s2 = s1;
+        s1 = s0;
+        s0 = y;
+        ds = (ds << 4) | static_cast<uint32_t>(d + 1);
+        ++n;
+      }
     }
-    if (g1 >= 0) {
-      Rec dummy;
-      (void)visit_child(v, g1, g[1], 0, sc, &dummy);
-      (void)visit_child(v, g1, g[1], 1, sc, &dummy);
-    }
+    if (n == 0) break;
+    x = s0;
+    d = static_cast<int>(ds & 15u);
+    s0 = s1;
+    s1 = s2;
+    s2 = s3;
+    ds >>= 4;
+    --n;
+    rx = v.R(x);
   }
 }
 
@@ -232,9 +246,9 @@ struct NodeOut {
     const uint32_t sh = static_cast<uint32_t>(level - root_level) & 31u;
     w[12] = __float_as_int(rg / static_cast<float>(static_cast<int32_t>(1u << sh)));
   }
-  __device__ __forceinline__ void children(const SlotChildren& sc) {
+  __device__ __forceinline__ void children(const View& v, const SlotChildren& sc) {
 #pragma unroll
-    for (int c = 0; c < 8; ++c) w[c] = sc.val[c];
+    for (int c = 0; c < 8; ++c) w[c] = key_value(v, sc.key(c));
     w[13] = static_cast<int32_t>(sc.node_mask);
     w[14] = 0;
   }
@@ -260,7 +274,7 @@ __device__ __forceinline__ void store_zeros(OctNode* nodes, int32_t from, int32_
 __device__ __forceinline__ void build_root(const View& v, int32_t root_level) {
   const SegBind& b = v.b;
   SlotChildren sc;
-  sc.init();
+  sc.init(v.sc_col);
   if (b.m >= 1) {
     // Node 0's own leaves link into the root slot.
     const Rec r0 = v.R(0);
@@ -269,7 +283,7 @@ __device__ __forceinline__ void build_root(const View& v, int32_t root_level) {
       if (r0.leaf(side)) {
         const int32_t leaf = r0.child() + side;
         if (leaf >= 0 && leaf <= b.m)
-          sc.put(child_of(v.CODE(leaf), r0.pn / 3 + 1), link_key(0, side), leaf);
+          sc.put(child_of(v.CODE(leaf), r0.pn / 3 + 1), link_key(0, side));
       }
     }
     // The group grows through the true root's internal children, the nodes
@@ -298,24 +312,17 @@ __device__ __forceinline__ void build_root(const View& v, int32_t root_level) {
         mr[q] = ry;
       } else if (ry.e > 0) {
         const uint32_t w = child_of(v.CODE(y), ry.pn / 3 - ry.e + 1);
-        sc.put(w, make_key(y, ry.e - 1), ry.p + ry.e - 1);
+        sc.put(w, make_key(y, ry.e - 1));
         sc.node_mask |= 1u << w;
       }
     }
-    // Members at depth 1 and below, like any group.
-    int32_t l2[4] = {-1, -1, -1, -1};
-    Rec q2[4];
+    // Members at depth 1 and below, down to depth 4.
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      if (mem[q] < 0) continue;
-      l2[2 * q] = visit_child(v, mem[q], mr[q], 0, sc, &q2[2 * q]);
-      l2[2 * q + 1] = visit_child(v, mem[q], mr[q], 1, sc, &q2[2 * q + 1]);
-    }
-    visit_members(v, l2[0], q2[0], l2[1], q2[1], sc);
-    visit_members(v, l2[2], q2[2], l2[3], q2[3], sc);
+    for (int q = 0; q < 2; ++q)
+      if (mem[q] >= 0) walk_group(v, mem[q], mr[q], 1, 4, sc);
   }
   NodeOut nd;
-  nd.children(sc);
+  nd.children(v, sc);
   const uint32_t shift = static_cast<uint32_t>(kMortonBits - 3 * root_level) & 31u;
   const uint32_t c = (v.CODE(0) >> shift) << shift;
   nd.w[8] = __float_as_int(decode_axis(c, 0, b.mn, b.rg));
@@ -383,7 +390,7 @@ __device__ __forceinline__ void build_chain(const View& v, int32_t i, int32_t ro
   const int32_t lvl_i = ri.pn / 3;
   StepBits bits;
   SlotChildren sc;
-  sc.init();
+  sc.init(v.sc_col);
   for (int32_t t = 0; t + 1 < nn; ++t) bits.set(t + 1, child_of(code_i, lvl_i - t));
   int32_t reach = p_i + nn;
   const bool before_matters = reach_before > p_i;
@@ -402,7 +409,7 @@ __device__ __forceinline__ void build_chain(const View& v, int32_t i, int32_t ro
       const int32_t t = s - pq - 1;  // chain q's step t writes slot pq+t+1
       const uint32_t w = child_of(code_q, lvl_q - t);
       if (s == p_i) {
-        sc.put(w, make_key(q, t), s - 1);
+        sc.put(w, make_key(q, t));
         sc.node_mask |= 1u << w;
       } else {
         bits.set(s - p_i, w);
@@ -412,13 +419,10 @@ __device__ __forceinline__ void build_chain(const View& v, int32_t i, int32_t ro
 
   // First slot: the earlier chains' step links, then the group's top links
   // and leaf links.
-  Rec r0, r1;
-  const int32_t m0 = visit_child(v, i, ri, 0, sc, &r0);
-  const int32_t m1 = visit_child(v, i, ri, 1, sc, &r1);
-  if (m0 >= 0 || m1 >= 0) visit_members(v, m0, r0, m1, r1, sc);
+  walk_group(v, i, ri, 0, 3, sc);
   if (p_i > 0 && p_i < b.k) {
     NodeOut nd;
-    nd.children(sc);
+    nd.children(v, sc);
     nd.geometry(code_i, lvl_i, root_level, b.mn, b.rg);
     nd.store(b.nodes + p_i);
   }
@@ -443,6 +447,7 @@ __global__ __launch_bounds__(kThreads) void oct_build_kernel(
   __shared__ v4i s_rec[kWinN];
   __shared__ uint32_t s_code[kWinN];
   __shared__ int32_t s_list[kNodeTile];
+  __shared__ uint32_t s_sc[8 * kThreads];
   __shared__ int32_t s_count;
   __shared__ int32_t s_emax;
 
@@ -511,7 +516,8 @@ __global__ __launch_bounds__(kThreads) void oct_build_kernel(
   emax = wave_incl_max(emax);
   if (lane_id() == 0) atomicMax(&s_emax, emax);
   __syncthreads();
-  const View v{b, lo, wn, wnc, as_lds<const v4i>(s_rec), as_lds<const uint32_t>(s_code)};
+  const View v{b, lo, wn, wnc, as_lds<const v4i>(s_rec), as_lds<const uint32_t>(s_code),
+               as_lds<uint32_t>(s_sc + tid)};
   // Compact the block's work: the root and the non-empty chains.
 #pragma unroll
   for (int h = 0; h < kNodeTile / kThreads; ++h) {
