@@ -45,29 +45,40 @@ struct Codes {
   __device__ __forceinline__ uint32_t operator[](int32_t q) const {
     return static_cast<uint32_t>(q - lo) < static_cast<uint32_t>(n) ? w[q - lo] : g[q];
   }
+  // For an index known to lie in the window: no fallback branch.
+  __device__ __forceinline__ uint32_t local(int32_t q) const { return w[q - lo]; }
 };
 
 #ifndef OCP_RADIX_HALO
 #define OCP_RADIX_HALO 256
 #endif
 constexpr int kHalo = OCP_RADIX_HALO;  // codes staged on either side of a tile
+// Nodes whose exponential search is still running at this length are
+// finished by a whole wave; the rest stay inside the LDS window.
+constexpr int32_t kLocalMax = kHalo / 2;
+constexpr int kLocalLog = __builtin_ctz(kLocalMax);
+static_assert((kLocalMax & (kLocalMax - 1)) == 0 && kLocalLog >= 1 && kLocalLog < 31,
+              "the local search bound is a power of two");
 
 // Split search of 04_radix_tree.cpp:94-104 for node i with far end j. The
 // reference's divisor runs over 2, 4, ..., 2^log2_ceil(l) and its step is
 // ceil(l / divisor); with power-of-two divisors that is a shift, so no
 // integer division is left on the device.
-__device__ __forceinline__ int32_t split_search(const Codes& codes,
+// Only called for local nodes (l < kLocalMax), whose probes stay in the window.
+__device__ __forceinline__ int32_t split_search(const Codes& codes, uint32_t code_i,
                                                 int32_t i, int32_t d,
                                                 int32_t l, int32_t n,
                                                 uint32_t delta_node) {
-  const uint32_t code_i = codes[i];
   int32_t s = 0;
   const int32_t lg = log2_ceil(static_cast<uint32_t>(l));
   const int32_t s_cutoff = (d == -1) ? i - 1 : n - i - 1;
-  for (int32_t k = 1; k <= lg; ++k) {
+  // l < kLocalMax, so lg <= kLocalLog: a fixed number of guarded steps.
+#pragma unroll
+  for (int32_t k = 1; k <= kLocalLog; ++k) {
     const int32_t t = static_cast<int32_t>((static_cast<uint32_t>(l) + (1u << k) - 1u) >> k);
     const int32_t st = s + t;
-    if (st <= s_cutoff && delta(code_i, codes[i + st * d]) > delta_node) s = st;
+    const bool probe = k <= lg && st <= s_cutoff;
+    if (probe && delta(code_i, codes.local(probe ? i + st * d : i)) > delta_node) s = st;
   }
   return i + s * d + min(d, 0);
 }
@@ -139,9 +150,6 @@ __device__ __forceinline__ NodeOut wave_node(const Codes& codes, int32_t i, int3
   return o;
 }
 
-// Nodes whose exponential search is still running at this length are
-// finished by a whole wave; the rest stay inside the LDS window.
-constexpr int32_t kLocalMax = kHalo / 2;
 
 __global__ __launch_bounds__(256) void radix_tree_kernel(
     const uint32_t* __restrict__ codes_all, uint8_t* __restrict__ prefix_all,
@@ -201,28 +209,45 @@ __global__ __launch_bounds__(256) void radix_tree_kernel(
     if (i == 0) {
       wide = true;
     } else {
-      const uint32_t code_i = codes[i];
-      const int32_t diff = static_cast<int32_t>(delta(code_i, codes[i + 1])) -
-                           static_cast<int32_t>(delta(code_i, codes[i - 1]));
+      // Every probe of a local node lies within kLocalMax of i (and inside
+      // [0, n] by the bound checks), and the window spans kHalo >= kLocalMax
+      // around the tile, so these reads skip the window test.
+      const uint32_t code_i = codes.local(i);
+      const uint32_t dn = delta(code_i, codes.local(i + 1));
+      const uint32_t dp = delta(code_i, codes.local(i - 1));
+      const int32_t diff = static_cast<int32_t>(dn) - static_cast<int32_t>(dp);
       const int32_t d = (diff > 0) - (diff < 0);
-      const uint32_t delta_min = delta(code_i, codes[i - d]);
+      // delta(code_i, codes[i - d]); d == 0 (equal neighbours, repeated
+      // codes only) compares code i with itself.
+      const uint32_t delta_min = d > 0 ? dp : (d < 0 ? dn : delta(code_i, code_i));
       // Exponential search for an upper bound l_max (a power of two), then
-      // binary search of the length: t = l_max / 2, l_max / 4, ..., 1.
-      int32_t l_max = 2;
-      while (l_max <= kLocalMax && i + l_max * d >= 0 && i + l_max * d <= n &&
-             delta(code_i, codes[i + l_max * d]) > delta_min)
-        l_max *= 2;
+      // binary search of the length: t = l_max / 2, l_max / 4, ..., 1. The
+      // reference probes 2, 4, 8, ... until one fails; here the probes up to
+      // kLocalMax are read at once and l_max = 2^(leading passes + 1).
+      uint32_t pass = 0;
+#pragma unroll
+      for (int k = 1; k <= kLocalLog; ++k) {
+        const int32_t q = i + (1 << k) * d;
+        const bool in = q >= 0 && q <= n;
+        const uint32_t cq = codes.local(in ? q : i);
+        pass |= static_cast<uint32_t>(in && delta(code_i, cq) > delta_min) << (k - 1);
+      }
+      const int32_t l_max = 2 << __builtin_ctz(~pass);
       if (l_max > kLocalMax) {
         wide = true;
       } else {
         const int32_t l_cutoff = (d == -1) ? i : n - i;
         int32_t l = 0;
-        for (int32_t t = l_max >> 1; t >= 1; t >>= 1) {
-          if (l + t <= l_cutoff && delta(code_i, codes[i + (l + t) * d]) > delta_min) l += t;
+#pragma unroll
+        for (int k = 1; k <= kLocalLog; ++k) {  // t = l_max / 2, l_max / 4, ..., 1
+          const int32_t t = l_max >> k;
+          const bool probe = t >= 1 && l + t <= l_cutoff;
+          if (probe && delta(code_i, codes.local(probe ? i + (l + t) * d : i)) > delta_min)
+            l += t;
         }
         const int32_t j = i + l * d;
-        const uint32_t delta_node = delta(code_i, codes[j]);
-        emit(i, delta_node, split_search(codes, i, d, l, n, delta_node), j);
+        const uint32_t delta_node = delta(code_i, codes.local(j));
+        emit(i, delta_node, split_search(codes, code_i, i, d, l, n, delta_node), j);
       }
     }
   }
