@@ -120,6 +120,19 @@ struct View {
   __device__ __forceinline__ uint32_t CODE(int32_t q) const {
     return static_cast<uint32_t>(q - lo) < static_cast<uint32_t>(nc) ? code[q - lo] : b.codes[q];
   }
+  // Record and code of internal node q (q < m) under one window test: the
+  // code window is at least as long as the node window.
+  __device__ __forceinline__ Rec RC(int32_t q, uint32_t* c) const {
+    if (static_cast<uint32_t>(q - lo) < static_cast<uint32_t>(n)) {
+      const v4i r = rec[q - lo];
+      *c = code[q - lo];
+      return {r.x, r.y, static_cast<uint32_t>(r.z), r.w};
+    }
+    *c = b.codes[q];
+    const uint32_t l = static_cast<uint32_t>(b.lc[q]) | (b.hl[q] ? kLeafL : 0u) |
+                       (b.hr[q] ? kLeafR : 0u);
+    return {b.p[q], b.e[q], l, static_cast<int32_t>(b.pn[q])};
+  }
 };
 
 __device__ __forceinline__ uint32_t child_of(uint32_t code, int32_t level) {
@@ -180,13 +193,14 @@ __device__ __forceinline__ int32_t visit_child(const View& v, int32_t x, const R
     return -1;
   }
   if (ch <= 0 || ch >= v.b.m) return -1;
-  const Rec ry = v.R(ch);
+  uint32_t code_ch;
+  const Rec ry = v.RC(ch, &code_ch);
   if (ry.e == 0) {
     *out = ry;
     return ch;
   }
   if (ry.e > 0) {
-    const uint32_t w = child_of(v.CODE(ch), ry.pn / 3 - ry.e + 1);
+    const uint32_t w = child_of(code_ch, ry.pn / 3 - ry.e + 1);
     sc.put(w, make_key(ch, ry.e - 1));
     sc.node_mask |= 1u << w;
   }
@@ -368,18 +382,33 @@ __device__ __forceinline__ void build_chain(const View& v, int32_t i, int32_t ro
   uint32_t owned = (1u << nn) - 1u;
   int32_t next_start = b.k;
   bool found_next = false;
-  for (int32_t q = i + 1; q < b.m; ++q) {
-    const int32_t pq = v.P(q);
-    if (found_next && pq > last) break;
-    const int32_t eq = v.E(q);
-    if (eq <= 0) continue;
+  auto fwd = [&](int32_t pq, int32_t eq) {  // one step; true ends the scan
+    if (found_next && pq > last) return true;
+    if (eq <= 0) return false;
     if (!found_next) {
       next_start = pq;
       found_next = true;
     }
-    if (pq > last) break;
+    if (pq > last) return true;
     const int32_t hi = min(pq + eq - 1, last);
     for (int32_t s = pq; s <= hi; ++s) owned &= ~(1u << (s - p_i));
+    return false;
+  };
+  {
+    // Inside the window the scan reads one LDS record per step; past its
+    // end it continues with the general lookups.
+    int32_t q = i + 1;
+    bool ended = false;
+    for (const int32_t q_win = min(b.m, v.lo + v.n); q < q_win; ++q) {
+      const v4i r = v.rec[q - v.lo];
+      if (fwd(r.x, r.y)) {
+        ended = true;
+        break;
+      }
+    }
+    if (!ended)
+      for (; q < b.m; ++q)
+        if (fwd(v.P(q), v.E(q))) break;
   }
 
   // Step links into this chain's slots: its own, and those of earlier chains
@@ -394,16 +423,8 @@ __device__ __forceinline__ void build_chain(const View& v, int32_t i, int32_t ro
   for (int32_t t = 0; t + 1 < nn; ++t) bits.set(t + 1, child_of(code_i, lvl_i - t));
   int32_t reach = p_i + nn;
   const bool before_matters = reach_before > p_i;
-  for (int32_t q = i - 1; q >= 1; --q) {
-    const int32_t pq = v.P(q);
-    const bool inside = q >= v.lo;
-    if (inside ? (pq + emax_win <= p_i && !before_matters) : (pq + kMaxEdge <= p_i)) break;
-    const int32_t eq = min(v.E(q), kMaxEdge);
-    if (eq <= 0) continue;
-    reach = max(reach, pq + eq);
-    if (pq + eq - 1 < p_i) continue;
-    const uint32_t code_q = v.CODE(q);
-    const int32_t lvl_q = v.R(q).pn / 3;
+  // Links of chain q (first slot pq, eq steps) that cover slots of chain i.
+  auto back_links = [&](int32_t q, int32_t pq, int32_t eq, uint32_t code_q, int32_t lvl_q) {
     const int32_t s_hi = min(pq + eq - 1, last);
     for (int32_t s = max(p_i, pq + 1); s <= s_hi; ++s) {
       const int32_t t = s - pq - 1;  // chain q's step t writes slot pq+t+1
@@ -413,6 +434,34 @@ __device__ __forceinline__ void build_chain(const View& v, int32_t i, int32_t ro
         sc.node_mask |= 1u << w;
       } else {
         bits.set(s - p_i, w);
+      }
+    }
+  };
+  {
+    int32_t q = i - 1;
+    bool ended = false;
+    for (const int32_t q_win = max(1, v.lo); q >= q_win; --q) {  // inside the window
+      const v4i r = v.rec[q - v.lo];
+      const int32_t pq = r.x;
+      if (pq + emax_win <= p_i && !before_matters) {
+        ended = true;
+        break;
+      }
+      const int32_t eq = min(r.y, kMaxEdge);
+      if (eq <= 0) continue;
+      reach = max(reach, pq + eq);
+      if (pq + eq - 1 < p_i) continue;
+      back_links(q, pq, eq, v.code[q - v.lo], r.w / 3);
+    }
+    if (!ended) {
+      for (; q >= 1; --q) {  // before the window
+        const int32_t pq = v.P(q);
+        if (pq + kMaxEdge <= p_i) break;
+        const int32_t eq = min(v.E(q), kMaxEdge);
+        if (eq <= 0) continue;
+        reach = max(reach, pq + eq);
+        if (pq + eq - 1 < p_i) continue;
+        back_links(q, pq, eq, v.CODE(q), v.R(q).pn / 3);
       }
     }
   }
