@@ -35,7 +35,7 @@
 //
 // Layout of the work: a workgroup of 128 threads takes 256 consecutive BRT
 // nodes and stages a packed 16-byte record {P, e, left child + leaf flags,
-// prefix length} and the code of those nodes plus 64 on either side into
+// prefix length} and the code of those nodes plus 32 on either side into
 // LDS with coalesced loads. It compacts the non-empty chains of its range
 // so that every lane processes a chain. The short walks over neighbouring
 // chains and most child lookups of the group walk are LDS reads (anything
@@ -58,9 +58,12 @@ namespace {
 constexpr int32_t kMaxEdge = 10;
 constexpr int kThreads = 128;
 #ifndef OCP_OCT_HALO
-#define OCP_OCT_HALO 64
+#define OCP_OCT_HALO 32
 #endif
-constexpr int kHalo = OCP_OCT_HALO;  // BRT nodes staged on either side of a tile
+// BRT nodes staged on either side of a tile. LDS per workgroup bounds the
+// waves per CU here: 10M uniform points, halo 8 -> 330 us, 16 -> 308,
+// 32 -> 286, 64 -> 296, 128 -> 340, 256 -> 422.
+constexpr int kHalo = OCP_OCT_HALO;
 constexpr int kWinN = kNodeTile + 2 * kHalo;
 constexpr uint32_t kLeafL = 1u << 30;
 constexpr uint32_t kLeafR = 1u << 31;
@@ -495,7 +498,8 @@ __global__ __launch_bounds__(kThreads) void oct_build_kernel(
     OctArgs a, SegView sv, const int32_t* __restrict__ tile_start) {
   __shared__ v4i s_rec[kWinN];
   __shared__ uint32_t s_code[kWinN];
-  __shared__ int32_t s_list[kNodeTile];
+  static_assert(kNodeTile <= 256, "work items are stored as bytes");
+  __shared__ uint8_t s_list[kNodeTile];  // work items as i - i0
   __shared__ uint32_t s_sc[8 * kThreads];
   __shared__ int32_t s_count;
   __shared__ int32_t s_emax;
@@ -577,7 +581,7 @@ __global__ __launch_bounds__(kThreads) void oct_build_kernel(
     if (lane_id() == 0 && bal)
       base = atomicAdd(reinterpret_cast<uint32_t*>(&s_count), static_cast<uint32_t>(__popcll(bal)));
     base = __shfl(base, 0, 64);
-    if (work) s_list[base + __popcll(bal & lanemask_lt())] = i;
+    if (work) s_list[base + __popcll(bal & lanemask_lt())] = static_cast<uint8_t>(i - i0);
   }
   __syncthreads();
   const int32_t root_level = b.m >= 1 ? v.R(0).pn / 3 : 0;
@@ -587,7 +591,7 @@ __global__ __launch_bounds__(kThreads) void oct_build_kernel(
   const int32_t reach_before = (lo > 0 && wn > 0) ? v.P(lo) + kMaxEdge : 0;
   const int32_t cnt = s_count;
   for (int q = tid; q < cnt; q += kThreads) {
-    const int32_t node = s_list[q];
+    const int32_t node = i0 + static_cast<int32_t>(s_list[q]);
     if (node == 0) build_root(v, root_level);
     else build_chain(v, node, root_level, emax_win, reach_before);
   }
