@@ -448,8 +448,9 @@ void enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
   uint32_t* dst = w.keys_b;
   for (int pass = 0; pass < 4; ++pass) {
     KScope k(c, kKPass0 + pass, capturing);
+    // The first pass may reorder equal digits: the keys carry no payload.
     launch_onesweep_pass(src, dst, p.sv, p.ts_sort, p.tiles_sort, w.hist, w.flags,
-                         c->ctrl, pass, tt, s);
+                         c->ctrl, pass, pass != 0, tt, s);
     std::swap(src, dst);
   }
   mark(c, 2, capturing);
@@ -714,8 +715,8 @@ int ocp_k_DigitBinning(ocp_ctx c, const uint32_t* hist, const uint32_t* sort, ui
   if (int r = ensure_scratch(c, int64_t(tiles) * 256)) return r;
   launch_begin(c->ctrl, nullptr, 0, c->stream);
   launch_onesweep_pass(sort, alt, single_view(size, 0.0f, 1.0f), nullptr, tiles, hist,
-                       c->s_flags, c->ctrl, static_cast<int>(radix_shift >> 3), c->tune,
-                       c->stream);
+                       c->s_flags, c->ctrl, static_cast<int>(radix_shift >> 3),
+                       /*stable=*/true, c->tune, c->stream);
   OCP_CHECK_LAUNCH(c);
   return OCP_OK;
 }
@@ -735,7 +736,7 @@ int ocp_k_SortKeys(ocp_ctx c, uint32_t* keys, uint32_t* alt, uint32_t n) {
   uint32_t* dst = alt;
   for (int pass = 0; pass < 4; ++pass) {
     launch_onesweep_pass(src, dst, sv, nullptr, tiles, c->s_hist, c->s_flags, c->ctrl, pass,
-                         c->tune, c->stream);
+                         pass != 0, c->tune, c->stream);
     std::swap(src, dst);
   }
   OCP_CHECK_LAUNCH(c);
@@ -1398,7 +1399,7 @@ int ocp_sort_learned(ocp_ctx c, uint32_t* keys, uint32_t* alt, uint32_t n,
     uint32_t* dst = alt;
     for (int pass = 0; pass < 4; ++pass) {
       launch_onesweep_pass(src, dst, sv, tb.big_tiles, big_tiles, tb.big_hist, c->s_flags,
-                           c->ctrl, pass, c->tune, c->stream);
+                           c->ctrl, pass, pass != 0, c->tune, c->stream);
       std::swap(src, dst);
     }
   }

@@ -844,7 +844,7 @@ void launch_lpass_a(const uint32_t* in, uint32_t* out, int32_t n, const LearnTab
   sv.n_seg = 1;
   sv.n1 = n;
   const int32_t tiles = (n + kLearnPassTile - 1) / kLearnPassTile;
-  onesweep_kernel<kLearnPassWaves, kLearnPassKpt, CoarseDigit>
+  onesweep_kernel<kLearnPassWaves, kLearnPassKpt, CoarseDigit, false>
       <<<tiles, kLearnPassWaves * 64, 0, s>>>(in, out, sv, nullptr, nullptr, tb.hist_a, 0, flags,
                                                ctrl, kStageBucket, CoarseDigit{tb});
 }
@@ -860,7 +860,7 @@ void launch_lpass_b(const uint32_t* in, uint32_t* out, int32_t n, const LearnTab
   sv.n_seg = kLearnCoarse;
   sv.start = tb.seg_start;
   sv.n1 = n;
-  onesweep_kernel<kLearnPassWaves, kLearnPassKpt, FineDigit>
+  onesweep_kernel<kLearnPassWaves, kLearnPassKpt, FineDigit, false>
       <<<learn_pass_b_max_tiles(n), kLearnPassWaves * 64, 0, s>>>(
           in, out, sv, tb.tile_start, tb.tiles_b, tb.hist_b, 0, flags, ctrl, kStageBucket + 1,
           FineDigit{tb});

@@ -79,10 +79,15 @@ template <int WAVES, int KPT>
 void launch_cfg(const uint32_t* in, uint32_t* out, const SegView& sv,
                 const int32_t* tile_start, int32_t total_tiles,
                 const uint32_t* hist, uint64_t* flags, uint32_t* ctrl,
-                int pass, hipStream_t s) {
-  onesweep_kernel<WAVES, KPT, RadixDigit><<<total_tiles, WAVES * 64, 0, s>>>(
-      in, out, sv, tile_start, nullptr, hist, pass, flags, ctrl,
-      kStagePass0 + static_cast<uint32_t>(pass), RadixDigit{8u * static_cast<uint32_t>(pass)});
+                int pass, bool stable, hipStream_t s) {
+  const uint32_t stage = kStagePass0 + static_cast<uint32_t>(pass);
+  const RadixDigit dig{8u * static_cast<uint32_t>(pass)};
+  if (stable)
+    onesweep_kernel<WAVES, KPT, RadixDigit, true><<<total_tiles, WAVES * 64, 0, s>>>(
+        in, out, sv, tile_start, nullptr, hist, pass, flags, ctrl, stage, dig);
+  else
+    onesweep_kernel<WAVES, KPT, RadixDigit, false><<<total_tiles, WAVES * 64, 0, s>>>(
+        in, out, sv, tile_start, nullptr, hist, pass, flags, ctrl, stage, dig);
 }
 
 }  // namespace
@@ -96,13 +101,13 @@ bool sort_config_supported(int waves, int kpt) {
 void launch_onesweep_pass(const uint32_t* in, uint32_t* out,
                           const SegView& sv, const int32_t* tile_start,
                           int32_t total_tiles, const uint32_t* hist,
-                          uint64_t* flags, uint32_t* ctrl, int pass,
+                          uint64_t* flags, uint32_t* ctrl, int pass, bool stable,
                           const TuneTable& tt, hipStream_t s) {
   if (total_tiles <= 0) return;
 #define OCP_SORT_CFG(W, K)                                                   \
   if (tt.sort_waves == W && tt.sort_kpt == K) {                              \
     launch_cfg<W, K>(in, out, sv, tile_start, total_tiles, hist, flags, ctrl, \
-                     pass, s);                                               \
+                     pass, stable, s);                                       \
     return;                                                                  \
   }
   OCP_SORT_CFG(4, 16)
@@ -113,7 +118,7 @@ void launch_onesweep_pass(const uint32_t* in, uint32_t* out,
   OCP_SORT_CFG(16, 8)
 #undef OCP_SORT_CFG
   launch_cfg<8, 15>(in, out, sv, tile_start, total_tiles, hist, flags, ctrl,
-                    pass, s);
+                    pass, stable, s);
 }
 
 void launch_histogram(const uint32_t* keys, const SegView& sv,

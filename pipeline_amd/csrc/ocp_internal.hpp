@@ -77,10 +77,12 @@ void launch_histogram_tiles(const uint32_t* keys, const SegView& sv,
                             const int32_t* tile_start, int32_t total_tiles,
                             int tile, uint32_t* hist, const TuneTable& tt,
                             hipStream_t s);
+// stable = false: equal digits may leave in any order (the first pass of a
+// sort of bare keys); full tiles then rank with LDS atomics.
 void launch_onesweep_pass(const uint32_t* in, uint32_t* out,
                           const SegView& sv, const int32_t* tile_start,
                           int32_t total_tiles, const uint32_t* hist,
-                          uint64_t* flags, uint32_t* ctrl, int pass,
+                          uint64_t* flags, uint32_t* ctrl, int pass, bool stable,
                           const TuneTable& tt, hipStream_t s);
 
 // ---- stage 3 (k_unique.hip)

@@ -19,6 +19,13 @@
 //      equal-digit runs go to consecutive HBM addresses.
 // Segments sort independently: each has its own digit histogram and its own
 // lookback chain, and no tile holds keys of two segments.
+//
+// STABLE = false is for passes whose order among equal digits is free: the
+// first pass of an LSD sort of bare keys (its input order is arbitrary, so
+// the sorted result is the same) and scatters whose buckets are sorted
+// afterwards. Full tiles then rank each key with one LDS atomic add on the
+// wave's digit counter instead of the eight-ballot multisplit; partial tiles
+// keep the multisplit so that padding keys still rank last.
 #pragma once
 
 #include "ocp_internal.hpp"
@@ -47,7 +54,7 @@ struct RadixDigit {
 // hist: per segment 1024 words; this pass reads words [slot*256, slot*256+256)
 // of its segment. tiles_dev (optional): the number of real tiles, on device,
 // when the grid is sized for an upper bound.
-template <int WAVES, int KPT, class Digit>
+template <int WAVES, int KPT, class Digit, bool STABLE = true>
 __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
     const uint32_t* __restrict__ in, uint32_t* __restrict__ out, SegView sv,
     const int32_t* __restrict__ tile_start, const int32_t* __restrict__ tiles_dev,
@@ -95,21 +102,30 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
   // Wave-level multisplit. The read of the digit counter by every lane of a
   // group happens in the same LDS instruction, before the leader's update.
   uint32_t* wh = s_whist + w * 256;
-  const uint64_t lt = lanemask_lt();
+  if (STABLE || cnt < T) {
+    const uint64_t lt = lanemask_lt();
 #pragma unroll
-  for (int k = 0; k < KPT; ++k) {
-    const uint32_t d = dg[k];
-    uint64_t m = ~0ull;
+    for (int k = 0; k < KPT; ++k) {
+      const uint32_t d = dg[k];
+      uint64_t m = ~0ull;
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bal = __ballot(bit);
-      m &= bit ? bal : ~bal;
+      for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        m &= bit ? bal : ~bal;
+      }
+      const uint32_t rank = __popcll(m & lt);
+      const uint32_t old = wh[d];
+      if (rank == 0) wh[d] = old + __popcll(m);
+      off[k] = old + rank;
     }
-    const uint32_t rank = __popcll(m & lt);
-    const uint32_t old = wh[d];
-    if (rank == 0) wh[d] = old + __popcll(m);
-    off[k] = old + rank;
+  } else {
+    // Each lane takes the next free rank of its digit within the wave.
+    lds_ptr<uint32_t> whl = as_lds<uint32_t>(wh);
+#pragma unroll
+    for (int k = 0; k < KPT; ++k)
+      off[k] = __hip_atomic_fetch_add(whl + dg[k], 1u, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   __syncthreads();
 
