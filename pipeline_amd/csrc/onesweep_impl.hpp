@@ -87,16 +87,23 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
   uint32_t keys[KPT];
   uint32_t dg[KPT];
   uint32_t off[KPT];
-  const uint32_t* src = in + base;
+  const uint32_t* src = in + static_cast<int32_t>(w) * (64 * KPT) + static_cast<int32_t>(lane) + base;
+  if (cnt == T) {  // every tile but a segment's last: no per-key guards
 #pragma unroll
-  for (int k = 0; k < KPT; ++k) {
-    const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
-    keys[k] = idx < cnt ? src[idx] : 0xffffffffu;
-  }
+    for (int k = 0; k < KPT; ++k) keys[k] = src[k * 64];
 #pragma unroll
-  for (int k = 0; k < KPT; ++k) {
-    const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
-    dg[k] = idx < cnt ? dig(keys[k]) : 255u;  // padding keys rank last
+    for (int k = 0; k < KPT; ++k) dg[k] = dig(keys[k]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
+      keys[k] = idx < cnt ? src[k * 64] : 0xffffffffu;
+    }
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
+      dg[k] = idx < cnt ? dig(keys[k]) : 255u;  // padding keys rank last
+    }
   }
 
   // Wave-level multisplit. The read of the digit counter by every lane of a
