@@ -65,13 +65,12 @@ struct Plan {
 
 // Kernels of one build, in launch order (per-kernel event timing).
 enum KernelId : int {
-  kKBegin = 0, kKMorton, kKPass0, kKPass1, kKPass2, kKPass3, kKFill, kKUnique,
+  kKBegin = 0, kKMorton, kKPass0, kKPass1, kKPass2, kKPass3, kKUnique,
   kKRadixTree, kKEdgeScan, kKOctOffsets, kKOctBuild, kKernelCount
 };
 const char* const kKernelNames[kKernelCount] = {
     "begin", "morton_hist", "onesweep_pass0", "onesweep_pass1", "onesweep_pass2",
-    "onesweep_pass3", "fill_counts", "unique", "radix_tree", "edge_scan",
-    "oct_offsets", "octree"};
+    "onesweep_pass3", "unique", "radix_tree", "edge_scan", "oct_offsets", "octree"};
 
 struct LastBuild {
   bool valid = false;
@@ -439,8 +438,11 @@ void enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
   int32_t* n_unique = w.counts;
   int32_t* kalloc = w.counts + w.seg_cap;
   mark(c, 0, capturing);
+  // Begin also clears the per-segment U and K counts (unique leaves an
+  // empty segment's entries untouched).
   { KScope k(c, kKBegin, capturing);
-    launch_begin(c->ctrl, w.hist, int64_t(p.n_seg) * 1024, s); }
+    launch_begin(c->ctrl, w.hist, int64_t(p.n_seg) * 1024, s,
+                 reinterpret_cast<uint32_t*>(w.counts), int64_t(w.seg_cap) * 2); }
   { KScope k(c, kKMorton, capturing);
     launch_morton(pts, f3, w.keys_a, p.sv, p.ts_morton, p.tiles_morton, w.hist, tt, s); }
   mark(c, 1, capturing);
@@ -454,8 +456,6 @@ void enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
     std::swap(src, dst);
   }
   mark(c, 2, capturing);
-  { KScope k(c, kKFill, capturing);
-    launch_fill_u32(reinterpret_cast<uint32_t*>(w.counts), 0u, int64_t(w.seg_cap) * 2, s); }
   { KScope k(c, kKUnique, capturing);
     launch_unique(w.keys_a, w.keys_b, p.sv, p.ts_scan, p.tiles_scan, w.flags, c->ctrl,
                   n_unique, kalloc, tt, s); }

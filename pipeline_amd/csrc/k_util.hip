@@ -17,7 +17,8 @@ __global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* p,
 
 __global__ __launch_bounds__(256) void begin_kernel(uint32_t* ctrl,
                                                     uint32_t* hist,
-                                                    int64_t n_hist) {
+                                                    int64_t n_hist,
+                                                    uint32_t* z2, int64_t n2) {
   const int64_t tid = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (tid == 0) {
 #pragma unroll
@@ -28,6 +29,7 @@ __global__ __launch_bounds__(256) void begin_kernel(uint32_t* ctrl,
   }
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t i = tid; i < n_hist; i += stride) hist[i] = 0;
+  for (int64_t i = tid; i < n2; i += stride) z2[i] = 0;
 }
 
 // Exclusive scan of the per-segment slot counts into node offsets; one
@@ -97,10 +99,10 @@ void launch_fill_u32(uint32_t* p, uint32_t value, int64_t n_words,
 }
 
 void launch_begin(uint32_t* ctrl, uint32_t* hist, int64_t n_hist,
-                  hipStream_t s) {
-  const int64_t blocks =
-      std::max<int64_t>(1, std::min<int64_t>((n_hist + 255) / 256, 2048));
-  begin_kernel<<<static_cast<int>(blocks), 256, 0, s>>>(ctrl, hist, n_hist);
+                  hipStream_t s, uint32_t* z2, int64_t n2) {
+  const int64_t blocks = std::max<int64_t>(
+      1, std::min<int64_t>((std::max(n_hist, n2) + 255) / 256, 2048));
+  begin_kernel<<<static_cast<int>(blocks), 256, 0, s>>>(ctrl, hist, n_hist, z2, n2);
 }
 
 void launch_oct_offsets(const int32_t* kalloc, int32_t n_seg,

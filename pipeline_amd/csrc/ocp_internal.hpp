@@ -53,9 +53,10 @@ bool morton_config_supported(int ppt);
 void launch_copy(void* dst, const void* src, int64_t bytes, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t value, int64_t n_words,
                      hipStream_t s);
-// Zero tickets, bump epoch; optionally zero n_hist words of histogram.
+// Zero tickets, bump epoch; optionally zero n_hist words of histogram and
+// n2 words at z2.
 void launch_begin(uint32_t* ctrl, uint32_t* hist, int64_t n_hist,
-                  hipStream_t s);
+                  hipStream_t s, uint32_t* z2 = nullptr, int64_t n2 = 0);
 void launch_oct_offsets(const int32_t* kalloc, int32_t n_seg,
                         int32_t* oct_start, uint32_t* ctrl, int64_t k_cap,
                         hipStream_t s);
