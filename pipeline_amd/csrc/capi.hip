@@ -41,6 +41,7 @@ struct Workspace {
   uint8_t* hr = nullptr;
   int32_t* lc = nullptr;
   int32_t* par = nullptr;
+  uint8_t* ppn = nullptr;  // prefix_n of each node's parent (fused build)
   int32_t* edge = nullptr;
   int32_t* offs = nullptr;
   OctNode* nodes = nullptr;
@@ -184,7 +185,7 @@ int ensure_scratch(ocp_ctx c, int64_t flag_words) {
 
 void free_workspace(Workspace& w) {
   dev_free(w.keys_a); dev_free(w.keys_b); dev_free(w.prefix_n); dev_free(w.hl);
-  dev_free(w.hr); dev_free(w.lc); dev_free(w.par); dev_free(w.edge);
+  dev_free(w.hr); dev_free(w.lc); dev_free(w.par); dev_free(w.ppn); dev_free(w.edge);
   dev_free(w.offs); dev_free(w.nodes); dev_free(w.hist); dev_free(w.flags);
   dev_free(w.counts); dev_free(w.oct_start); dev_free(w.tables); dev_free(w.boxes);
   w = Workspace{};
@@ -213,12 +214,14 @@ int ensure_workspace(ocp_ctx c, int64_t n, int32_t n_seg, int64_t flag_words) {
   if (grow_n) {
     const int64_t cap = std::max<int64_t>(n, 4096);
     dev_free(w.keys_a); dev_free(w.keys_b); dev_free(w.prefix_n); dev_free(w.hl);
-    dev_free(w.hr); dev_free(w.lc); dev_free(w.par); dev_free(w.edge); dev_free(w.offs);
+    dev_free(w.hr); dev_free(w.lc); dev_free(w.par); dev_free(w.ppn); dev_free(w.edge);
+    dev_free(w.offs);
     int r = 0;
     if ((r = dev_alloc(c, &w.keys_a, cap)) || (r = dev_alloc(c, &w.keys_b, cap)) ||
         (r = dev_alloc(c, &w.prefix_n, cap)) || (r = dev_alloc(c, &w.hl, cap)) ||
         (r = dev_alloc(c, &w.hr, cap)) || (r = dev_alloc(c, &w.lc, cap)) ||
-        (r = dev_alloc(c, &w.par, cap)) || (r = dev_alloc(c, &w.edge, cap)) ||
+        (r = dev_alloc(c, &w.par, cap)) || (r = dev_alloc(c, &w.ppn, cap)) ||
+        (r = dev_alloc(c, &w.edge, cap)) ||
         (r = dev_alloc(c, &w.offs, cap)))
       return r;
     w.n_cap = cap;
@@ -461,11 +464,11 @@ void enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
                   n_unique, kalloc, tt, s); }
   mark(c, 3, capturing);
   { KScope k(c, kKRadixTree, capturing);
-    launch_radix_tree(w.keys_b, w.prefix_n, w.hl, w.hr, w.lc, w.par, p.sv, p.ts_node,
+    launch_radix_tree(w.keys_b, w.prefix_n, w.hl, w.hr, w.lc, w.par, w.ppn, p.sv, p.ts_node,
                       p.tiles_node, n_unique, 0, s); }
   mark(c, 4, capturing);
   { KScope k(c, kKEdgeScan, capturing);
-    launch_edge_scan(w.prefix_n, w.par, w.edge, w.offs, p.sv, p.ts_scan, p.tiles_scan,
+    launch_edge_scan(w.prefix_n, w.par, w.ppn, w.edge, w.offs, p.sv, p.ts_scan, p.tiles_scan,
                      n_unique, 0, w.flags, c->ctrl, kalloc, 0, tt, s); }
   mark(c, 5, capturing);
   enqueue_octree(c, p, capturing);
@@ -769,7 +772,7 @@ int ocp_k_BuildRadixTree(ocp_ctx c, int32_t n, const uint32_t* codes, uint8_t* p
   if (!codes || !prefix_n || !hl || !hr || !lc || !parent)
     return fail(c, OCP_ERR_INVALID, "null pointer");
   if (n > kMaxPoints) return fail(c, OCP_ERR_CAPACITY, "n > 2^30-1");
-  launch_radix_tree(codes, prefix_n, hl, hr, lc, parent, single_view(n + 1, 0.0f, 1.0f),
+  launch_radix_tree(codes, prefix_n, hl, hr, lc, parent, nullptr, single_view(n + 1, 0.0f, 1.0f),
                     nullptr, static_cast<int32_t>(cdiv(n, kNodeTile)), nullptr, n, c->stream);
   OCP_CHECK_LAUNCH(c);
   return OCP_OK;
@@ -780,7 +783,7 @@ int ocp_k_EdgeCount(ocp_ctx c, const uint8_t* prefix_n, const int32_t* parents,
   if (check_ctx(c)) return OCP_ERR_INVALID;
   if (n <= 0) return OCP_OK;
   if (!prefix_n || !parents || !edge_count) return fail(c, OCP_ERR_INVALID, "null pointer");
-  launch_edge_scan(prefix_n, parents, edge_count, nullptr, single_view(n, 0.0f, 1.0f),
+  launch_edge_scan(prefix_n, parents, nullptr, edge_count, nullptr, single_view(n, 0.0f, 1.0f),
                    nullptr, static_cast<int32_t>(cdiv(n, scan_tile(c->tune))), nullptr, n,
                    nullptr, c->ctrl, nullptr, 1, c->tune, c->stream);
   OCP_CHECK_LAUNCH(c);
@@ -797,7 +800,7 @@ int ocp_k_PartialSum(ocp_ctx c, const int32_t* data, int64_t first, int64_t last
   const int32_t tiles = static_cast<int32_t>(cdiv(n, scan_tile(c->tune)));
   if (int r = ensure_scratch(c, tiles)) return r;
   launch_begin(c->ctrl, nullptr, 0, c->stream);
-  launch_edge_scan(nullptr, nullptr, const_cast<int32_t*>(data + first), d_first,
+  launch_edge_scan(nullptr, nullptr, nullptr, const_cast<int32_t*>(data + first), d_first,
                    single_view(n, 0.0f, 1.0f), nullptr, tiles, nullptr,
                    static_cast<int32_t>(n), c->s_flags, c->ctrl, nullptr, 2, c->tune,
                    c->stream);

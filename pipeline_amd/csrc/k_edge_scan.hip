@@ -19,7 +19,7 @@ namespace {
 template <int IPT>
 __global__ __launch_bounds__(256) void edge_scan_kernel(
     const uint8_t* __restrict__ prefix_all, const int32_t* __restrict__ par_all,
-    int32_t* __restrict__ edge_all, int32_t* __restrict__ off_all, SegView sv,
+    const uint8_t* __restrict__ ppn_all, int32_t* __restrict__ edge_all, int32_t* __restrict__ off_all, SegView sv,
     const int32_t* __restrict__ tile_start, const int32_t* __restrict__ n_unique,
     int32_t n_brt_host, uint64_t* __restrict__ flags,
     uint32_t* __restrict__ ctrl, int32_t* __restrict__ kalloc, int mode) {
@@ -53,8 +53,10 @@ __global__ __launch_bounds__(256) void edge_scan_kernel(
       if (mode == 2) {
         v = edge[i];
       } else if (i > 0) {
-        const int32_t p = par_all[sb + i];
-        v = static_cast<int32_t>(pn[i] / 3) - static_cast<int32_t>(pn[p] / 3);
+        // prefix_n of the parent: stored beside the node by the radix tree
+        // in the fused build, else gathered through parent[].
+        const uint32_t ppn = ppn_all ? ppn_all[sb + i] : pn[par_all[sb + i]];
+        v = static_cast<int32_t>(pn[i] / 3) - static_cast<int32_t>(ppn / 3);
       }
       if (mode != 2) edge[i] = v;
     }
@@ -110,29 +112,30 @@ __global__ __launch_bounds__(256) void edge_scan_kernel(
 }
 
 template <int IPT>
-void launch_ipt(const uint8_t* prefix_n, const int32_t* parent, int32_t* edge,
-                int32_t* offsets, const SegView& sv, const int32_t* tile_start,
-                int32_t total_tiles, const int32_t* n_unique,
+void launch_ipt(const uint8_t* prefix_n, const int32_t* parent, const uint8_t* parent_pn,
+                int32_t* edge, int32_t* offsets, const SegView& sv,
+                const int32_t* tile_start, int32_t total_tiles, const int32_t* n_unique,
                 int32_t n_brt_host, uint64_t* flags, uint32_t* ctrl,
                 int32_t* kalloc, int mode, hipStream_t s) {
   edge_scan_kernel<IPT><<<total_tiles, 256, 0, s>>>(
-      prefix_n, parent, edge, offsets, sv, tile_start, n_unique, n_brt_host,
+      prefix_n, parent, parent_pn, edge, offsets, sv, tile_start, n_unique, n_brt_host,
       flags, ctrl, kalloc, mode);
 }
 
 }  // namespace
 
 void launch_edge_scan(const uint8_t* prefix_n, const int32_t* parent,
-                      int32_t* edge, int32_t* offsets, const SegView& sv,
+                      const uint8_t* parent_pn, int32_t* edge, int32_t* offsets,
+                      const SegView& sv,
                       const int32_t* tile_start, int32_t total_tiles,
                       const int32_t* n_unique, int32_t n_brt_host,
                       uint64_t* flags, uint32_t* ctrl, int32_t* kalloc,
                       int mode, const TuneTable& tt, hipStream_t s) {
   if (total_tiles <= 0) return;
   switch (tt.scan_ipt) {
-    case 32: launch_ipt<32>(prefix_n, parent, edge, offsets, sv, tile_start, total_tiles, n_unique, n_brt_host, flags, ctrl, kalloc, mode, s); break;
-    case 8: launch_ipt<8>(prefix_n, parent, edge, offsets, sv, tile_start, total_tiles, n_unique, n_brt_host, flags, ctrl, kalloc, mode, s); break;
-    default: launch_ipt<16>(prefix_n, parent, edge, offsets, sv, tile_start, total_tiles, n_unique, n_brt_host, flags, ctrl, kalloc, mode, s); break;
+    case 32: launch_ipt<32>(prefix_n, parent, parent_pn, edge, offsets, sv, tile_start, total_tiles, n_unique, n_brt_host, flags, ctrl, kalloc, mode, s); break;
+    case 8: launch_ipt<8>(prefix_n, parent, parent_pn, edge, offsets, sv, tile_start, total_tiles, n_unique, n_brt_host, flags, ctrl, kalloc, mode, s); break;
+    default: launch_ipt<16>(prefix_n, parent, parent_pn, edge, offsets, sv, tile_start, total_tiles, n_unique, n_brt_host, flags, ctrl, kalloc, mode, s); break;
   }
 }
 

@@ -17,7 +17,8 @@
 // true root split and stores 0 into parent[0] and into the parent entries
 // of the true root's internal children; nodes i >= 1 store plainly. The
 // result equals the sequential reference on zeroed memory without atomics
-// or a zero-fill pass.
+// or a zero-fill pass. With the same writes the fused build also stores
+// prefix_n[parent[i]] at parent_pn[i], so the edge count needs no gather.
 #include "ocp_internal.hpp"
 
 namespace ocp {
@@ -156,7 +157,8 @@ __device__ __forceinline__ NodeOut wave_node(const Codes& codes, int32_t i, int3
 __global__ __launch_bounds__(256) void radix_tree_kernel(
     const uint32_t* __restrict__ codes_all, uint8_t* __restrict__ prefix_all,
     uint8_t* __restrict__ hl_all, uint8_t* __restrict__ hr_all,
-    int32_t* __restrict__ lc_all, int32_t* __restrict__ par_all, SegView sv,
+    int32_t* __restrict__ lc_all, int32_t* __restrict__ par_all,
+    uint8_t* __restrict__ ppn_all, SegView sv,
     const int32_t* __restrict__ tile_start, const int32_t* __restrict__ n_unique,
     int32_t n_brt_host) {
   static_assert(kLocalMax <= kHalo / 2 + 1, "local searches must stay in the window");
@@ -169,6 +171,7 @@ __global__ __launch_bounds__(256) void radix_tree_kernel(
   if (i0 >= n) return;
   const int32_t sb = seg_begin(sv, r.seg);
   int32_t* parent = par_all + sb;
+  uint8_t* parent_pn = ppn_all ? ppn_all + sb : nullptr;
   if (threadIdx.x == 0) s_nwide = 0;
   // Stage codes [i0 - halo, i0 + kNodeTile + halo) of the n+1 codes.
   const int32_t lo = max(0, i0 - kHalo);
@@ -201,6 +204,10 @@ __global__ __launch_bounds__(256) void radix_tree_kernel(
     if (node > 0) {
       if (!leaf_l) parent[gamma] = node;
       if (!leaf_r) parent[gamma + 1] = node;
+      if (parent_pn) {
+        if (!leaf_l) parent_pn[gamma] = static_cast<uint8_t>(delta_node);
+        if (!leaf_r) parent_pn[gamma + 1] = static_cast<uint8_t>(delta_node);
+      }
     }
   };
 
@@ -278,6 +285,12 @@ __global__ __launch_bounds__(256) void radix_tree_kernel(
         parent[0] = 0;
         if (g0 != 0 && g0 < n) parent[g0] = 0;
         if (g0 + 1 != n && g0 + 1 < n) parent[g0 + 1] = 0;
+        if (parent_pn) {
+          const uint8_t pn0 = static_cast<uint8_t>(o.delta_node);
+          parent_pn[0] = pn0;
+          if (g0 != 0 && g0 < n) parent_pn[g0] = pn0;
+          if (g0 + 1 != n && g0 + 1 < n) parent_pn[g0 + 1] = pn0;
+        }
       }
     }
   }
@@ -287,14 +300,14 @@ __global__ __launch_bounds__(256) void radix_tree_kernel(
 
 void launch_radix_tree(const uint32_t* codes, uint8_t* prefix_n,
                        uint8_t* has_leaf_left, uint8_t* has_leaf_right,
-                       int32_t* left_child, int32_t* parent, const SegView& sv,
-                       const int32_t* tile_start, int32_t total_tiles,
+                       int32_t* left_child, int32_t* parent, uint8_t* parent_pn,
+                       const SegView& sv, const int32_t* tile_start, int32_t total_tiles,
                        const int32_t* n_unique, int32_t n_brt_host,
                        hipStream_t s) {
   if (total_tiles <= 0) return;
   static_assert(kNodeTile % 256 == 0, "whole node rows per thread");
   radix_tree_kernel<<<total_tiles, 256, 0, s>>>(
-      codes, prefix_n, has_leaf_left, has_leaf_right, left_child, parent, sv,
+      codes, prefix_n, has_leaf_left, has_leaf_right, left_child, parent, parent_pn, sv,
       tile_start, n_unique, n_brt_host);
 }
 

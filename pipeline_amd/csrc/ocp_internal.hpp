@@ -95,9 +95,11 @@ void launch_unique(const uint32_t* in, uint32_t* out, const SegView& sv,
 // ---- stage 4 (k_radix_tree.hip)
 // n_brt: per-segment node counts; when n_unique != nullptr the count of
 // segment k is n_unique[k]-1, else n_brt_host (single segment).
+// parent_pn (optional): prefix_n[parent[i]] at i, written with parent[].
 void launch_radix_tree(const uint32_t* codes, uint8_t* prefix_n,
                        uint8_t* has_leaf_left, uint8_t* has_leaf_right,
-                       int32_t* left_child, int32_t* parent, const SegView& sv,
+                       int32_t* left_child, int32_t* parent, uint8_t* parent_pn,
+                       const SegView& sv,
                        const int32_t* tile_start, int32_t total_tiles,
                        const int32_t* n_unique, int32_t n_brt_host,
                        hipStream_t s);
@@ -105,8 +107,10 @@ void launch_radix_tree(const uint32_t* codes, uint8_t* prefix_n,
 // ---- stages 5 + 6 (k_edge_scan.hip)
 // mode 0: edge count + inclusive scan fused; 1: edge count only;
 // 2: inclusive scan of `edge` only.
+// parent_pn (optional, fused build): used instead of prefix_n[parent[i]].
 void launch_edge_scan(const uint8_t* prefix_n, const int32_t* parent,
-                      int32_t* edge, int32_t* offsets, const SegView& sv,
+                      const uint8_t* parent_pn, int32_t* edge, int32_t* offsets,
+                      const SegView& sv,
                       const int32_t* tile_start, int32_t total_tiles,
                       const int32_t* n_unique, int32_t n_brt_host,
                       uint64_t* flags, uint32_t* ctrl, int32_t* kalloc,
