@@ -270,7 +270,7 @@ int tune_key(const TuneTable& t) {
 int64_t flag_words_for(const TuneTable& tt, int64_t n, int32_t n_seg) {
   const int64_t sort_tiles = cdiv(n, sort_tile(tt)) + n_seg;
   const int64_t scan_tiles = cdiv(n, scan_tile(tt)) + n_seg;
-  return std::max(sort_tiles * 256, scan_tiles);
+  return std::max(sort_tiles * 256, 2 * scan_tiles);  // scans: tile flags + group sums
 }
 
 // Plain plan: one segment, scalar box, no device tables.
@@ -757,7 +757,7 @@ int ocp_k_CountUnique(ocp_ctx c, uint32_t* keys, int32_t* num_unique_out, int32_
     return OCP_OK;
   }
   const int32_t tiles = static_cast<int32_t>(cdiv(n, scan_tile(c->tune)));
-  if (int r = ensure_scratch(c, tiles)) return r;
+  if (int r = ensure_scratch(c, 2 * int64_t(tiles))) return r;  // tile flags + group sums
   launch_begin(c->ctrl, nullptr, 0, c->stream);
   launch_unique(keys, keys, single_view(n, 0.0f, 1.0f), nullptr, tiles, c->s_flags, c->ctrl,
                 num_unique_out, nullptr, c->tune, c->stream);
@@ -798,7 +798,7 @@ int ocp_k_PartialSum(ocp_ctx c, const int32_t* data, int64_t first, int64_t last
   if (!data || !d_first || first < 0) return fail(c, OCP_ERR_INVALID, "bad arguments");
   if (n > kMaxPoints) return fail(c, OCP_ERR_CAPACITY, "n > 2^30-1");
   const int32_t tiles = static_cast<int32_t>(cdiv(n, scan_tile(c->tune)));
-  if (int r = ensure_scratch(c, tiles)) return r;
+  if (int r = ensure_scratch(c, 2 * int64_t(tiles))) return r;  // tile flags + group sums
   launch_begin(c->ctrl, nullptr, 0, c->stream);
   launch_edge_scan(nullptr, nullptr, nullptr, const_cast<int32_t*>(data + first), d_first,
                    single_view(n, 0.0f, 1.0f), nullptr, tiles, nullptr,

@@ -3,12 +3,14 @@
 // inclusive scan of kernels/06_partial_sum.cpp:3-15 (the reference GPU path
 // has no scan and round-trips to the host, apps/gpu_demo/main.cu:308-310).
 //
-// Tiles of 256*IPT nodes, one per workgroup, one decoupled-lookback chain
-// per segment. Each thread handles one node per row of 256 (coalesced loads of
-// prefix_n and parent, one gathered byte prefix_n[parent]); rows are scanned
-// with wave64 shuffles, one wave scans the 4*IPT row totals and looks back
-// 64 predecessors per round trip. Each tile also reduces max(P[i] + e[i]) into the segment's
-// octree slot count with one atomicMax.
+// Tiles of 256*IPT nodes, one per workgroup, one lookback chain per segment.
+// Each thread handles one node per row of 256 (coalesced loads of prefix_n
+// and, in the fused build, the parent's prefix_n stored by the radix tree;
+// else parent and one gathered byte); rows are scanned with wave64 shuffles,
+// one wave scans the 4*IPT row totals and does the two-level lookback (the
+// tile's group of 64, then the sums of earlier groups). Each tile also
+// reduces max(P[i] + e[i]) into the segment's octree slot count with one
+// atomicMax.
 //
 // Modes: 0 = fused, 1 = edge count only, 2 = scan of `edge` only.
 #include "ocp_internal.hpp"
@@ -75,13 +77,13 @@ __global__ __launch_bounds__(256) void edge_scan_kernel(
     const uint32_t total = wave_scan_lds<IPT * NW>(s_rows);
     const uint32_t tag = ctrl[kCtrlEpoch] * 8u + kStageScan;
     if (lane == 0)
-      flag_store(&flags[t], flag_pack(tag, r.local == 0 ? kFlagInc : kFlagAgg, total));
-    uint32_t prev = 0;
+      flag_store(&flags[t], flag_pack(tag, kFlagAgg, total));
     bool late = false;
-    if (r.local > 0) prev = wave_lookback(flags, t, t - r.local, tag, &late);
+    // Group sums live after the grid's tile flags.
+    const uint32_t prev =
+        wave_lookback_groups(flags, flags + gridDim.x, t, t - r.local, total, tag, &late);
     if (lane == 0) {
       if (late) ctrl[kCtrlTimeout] = 1u;
-      if (r.local > 0) flag_store(&flags[t], flag_pack(tag, kFlagInc, prev + total));
       s_excl = prev;
     }
   }

@@ -5,9 +5,10 @@
 // Tiles of 256*IPT keys, one per workgroup. A key is kept when it is the first
 // of its segment or differs from its predecessor. Per row of 64 keys a
 // ballot gives the keep mask, the lane's rank (mbcnt) and the row count; one
-// wave scans the 4*IPT row counts and does the decoupled lookback of the
-// tile total, 64 predecessors per round trip. Kept keys are compacted in LDS, then written with
-// consecutive threads on consecutive output slots. The segment's last tile
+// wave scans the 4*IPT row counts and does the two-level lookback of the
+// tile total (the tile's group of 64, then the sums of earlier groups). Kept
+// keys are compacted in LDS, then written with consecutive threads on
+// consecutive output slots. The segment's last tile
 // writes U and seeds the octree slot count (1 when U >= 1).
 //
 // In place (out == in) is safe: a tile reads all its keys, and its
@@ -76,13 +77,13 @@ __global__ __launch_bounds__(256) void unique_kernel(
   if (w == 0) {
     const uint32_t total = wave_scan_lds<IPT * NW>(s_rows);
     if (lane == 0)
-      flag_store(&flags[t], flag_pack(tag, r.local == 0 ? kFlagInc : kFlagAgg, total));
-    uint32_t prev = 0;
+      flag_store(&flags[t], flag_pack(tag, kFlagAgg, total));
     bool late = false;
-    if (r.local > 0) prev = wave_lookback(flags, t, t - r.local, tag, &late);
+    // Group sums live after the grid's tile flags.
+    const uint32_t prev =
+        wave_lookback_groups(flags, flags + gridDim.x, t, t - r.local, total, tag, &late);
     if (lane == 0) {
       if (late) ctrl[kCtrlTimeout] = 1u;
-      if (r.local > 0) flag_store(&flags[t], flag_pack(tag, kFlagInc, prev + total));
       s_excl = prev;
       s_total = total;
       const int32_t n_tiles = (se - sb + T - 1) / T;

@@ -212,48 +212,58 @@ __device__ __forceinline__ uint32_t lookback_n(const uint64_t* flags,
   return excl;
 }
 
-// Decoupled lookback of a single-value chain by a whole wave: lane l polls
-// tile k-1-l, so 64 predecessors cost one round trip. Every lane of the wave
-// must call it; all lanes return the exclusive prefix of tile t. `first` is
-// the chain's first tile (it publishes an inclusive prefix).
-__device__ __forceinline__ uint32_t wave_lookback(const uint64_t* flags, int32_t t,
-                                                  int32_t first, uint32_t tag,
-                                                  bool* timed_out) {
+// Sum of the counts of `cnt` (< 65) flags base[0], base[stride], ... by a
+// whole wave once every one of them carries `want`; bounded like every wait.
+__device__ __forceinline__ uint32_t wave_sum_flags(const uint64_t* base, int32_t stride,
+                                                   int32_t cnt, uint32_t want,
+                                                   bool* timed_out) {
   const int32_t lane = static_cast<int32_t>(lane_id());
-  const uint32_t want = tag & 0x3fffffffu;
-  uint32_t excl = 0;
-  int32_t k = t - 1;
-  for (uint32_t spin = 0; spin < kSpinLimit;) {
-    const int32_t kk = k - lane;
-    bool ready = true, inc = true;
-    uint32_t cnt = 0;
-    if (kk >= first) {
-      const uint64_t v = flag_load(flags + kk);
+  for (uint32_t spin = 0; spin < kSpinLimit; ++spin) {
+    bool ready = true;
+    uint32_t c = 0;
+    if (lane < cnt) {
+      const uint64_t v = flag_load(base + static_cast<int64_t>(lane) * stride);
       const uint32_t hi = static_cast<uint32_t>(v >> 32);
-      const uint32_t st = hi & 3u;
-      ready = (hi >> 2) == want && st != 0;
-      inc = ready && st == kFlagInc;
-      cnt = static_cast<uint32_t>(v);
+      ready = (hi >> 2) == want && (hi & 3u) != 0;
+      c = static_cast<uint32_t>(v);
     }
-    const uint64_t inc_mask = __ballot(inc);
-    const uint64_t nr_mask = __ballot(!ready);
-    const int f = inc_mask ? __ffsll(static_cast<unsigned long long>(inc_mask)) - 1 : 64;
-    const uint64_t upto = f >= 63 ? ~0ull : ((1ull << (f + 1)) - 1ull);
-    if (nr_mask & upto) {
+    if (__ballot(!ready)) {
       __builtin_amdgcn_s_sleep(1);
-      ++spin;
       continue;
     }
-    uint32_t part = (lane <= f) ? cnt : 0u;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, kWave);
-    excl += part;
-    if (f < 64) return excl;
-    k -= kWave;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, kWave);
+    return c;
   }
   *timed_out = true;
+  return 0;
+}
+
+// Two-level lookback of a single-value chain by a whole wave. Every tile
+// publishes its own count at flags[t] before calling; tiles are grouped by
+// 64 from the chain's first tile, and the last tile of a group also
+// publishes the group's sum at gflags[first tile of the group]. Tile t sums
+// the counts of its group's earlier tiles and the sums of all earlier
+// groups: two round trips (plus one per 64 groups) whatever its position,
+// where a chain of single-tile flags resolves about 128 tiles per round
+// trip. No wait depends on a later tile: the counts are published before
+// any wait, and a group's last tile waits only on its own group. Every lane
+// must call it; all lanes return the exclusive prefix of tile t.
+__device__ __forceinline__ uint32_t wave_lookback_groups(uint64_t* flags, uint64_t* gflags,
+                                                         int32_t t, int32_t first,
+                                                         uint32_t own, uint32_t tag,
+                                                         bool* timed_out) {
+  const uint32_t want = tag & 0x3fffffffu;
+  const int32_t local = t - first;
+  const int32_t g0 = local & ~63;  // this tile's group, as a local tile index
+  uint32_t excl = wave_sum_flags(flags + first + g0, 1, local - g0, want, timed_out);
+  if ((local & 63) == 63 && lane_id() == 0)
+    flag_store(gflags + first + g0, flag_pack(tag, kFlagAgg, excl + own));
+  for (int32_t g = 0; g < g0; g += 64 * 64)
+    excl += wave_sum_flags(gflags + first + g, 64, min(64, (g0 - g) >> 6), want, timed_out);
   return excl;
 }
+
 
 // ---------------------------------------------------------------- segments
 // A batch of independent clouds. `start` holds n_seg+1 point offsets; with
