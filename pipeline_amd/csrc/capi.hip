@@ -270,7 +270,8 @@ int tune_key(const TuneTable& t) {
 int64_t flag_words_for(const TuneTable& tt, int64_t n, int32_t n_seg) {
   const int64_t sort_tiles = cdiv(n, sort_tile(tt)) + n_seg;
   const int64_t scan_tiles = cdiv(n, scan_tile(tt)) + n_seg;
-  return std::max(sort_tiles * 256, 2 * scan_tiles);  // scans: tile flags + group sums
+  // scans: tile flags, group sums, tile maxima
+  return std::max(sort_tiles * 256, 3 * scan_tiles);
 }
 
 // Plain plan: one segment, scalar box, no device tables.

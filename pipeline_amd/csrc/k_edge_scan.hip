@@ -8,9 +8,9 @@
 // and, in the fused build, the parent's prefix_n stored by the radix tree;
 // else parent and one gathered byte); rows are scanned with wave64 shuffles,
 // one wave scans the 4*IPT row totals and does the two-level lookback (the
-// tile's group of 64, then the sums of earlier groups). Each tile also
-// reduces max(P[i] + e[i]) into the segment's octree slot count with one
-// atomicMax.
+// tile's group of 64, then the sums of earlier groups). The tiles' maxima of
+// P[i] + e[i] reach the segment's octree slot count with one atomicMax per
+// group of 64 tiles.
 //
 // Modes: 0 = fused, 1 = edge count only, 2 = scan of `edge` only.
 #include "ocp_internal.hpp"
@@ -104,11 +104,29 @@ __global__ __launch_bounds__(256) void edge_scan_kernel(
     kmax = wave_incl_max(kmax);
     if (lane == 0) s_max[w] = kmax;
     __syncthreads();
-    if (tid == 0) {
+    if (w == 0) {
       int32_t m = s_max[0];
 #pragma unroll
       for (int q = 1; q < NW; ++q) m = max(m, s_max[q]);
-      atomicMax(&kalloc[r.seg], m);
+      // Tile maxima reach K_alloc one group of 64 tiles at a time: the last
+      // tile of a group (or of the segment) folds in the maxima its group's
+      // earlier tiles published, so the segment's counter takes one atomic
+      // per group (same-address atomics from every tile serialise: 19 us at
+      // 10M). The waits go to earlier tiles only.
+      const uint32_t tag = ctrl[kCtrlEpoch] * 8u + kStageScan;
+      uint64_t* mflags = flags + 2 * static_cast<int64_t>(gridDim.x);
+      if (lane == 0) flag_store(mflags + t, flag_pack(tag, kFlagAgg, static_cast<uint32_t>(m)));
+      const int32_t local = r.local;
+      const int32_t g0 = local & ~63;
+      if ((local & 63) == 63 || local == (n + T - 1) / T - 1) {
+        bool late = false;
+        const uint32_t gm = wave_sum_flags<true>(mflags + (t - local) + g0, 1, local - g0,
+                                                 tag & 0x3fffffffu, &late);
+        if (lane == 0) {
+          if (late) ctrl[kCtrlTimeout] = 1u;
+          atomicMax(&kalloc[r.seg], max(m, static_cast<int32_t>(gm)));
+        }
+      }
     }
   }
 }

@@ -212,8 +212,10 @@ __device__ __forceinline__ uint32_t lookback_n(const uint64_t* flags,
   return excl;
 }
 
-// Sum of the counts of `cnt` (< 65) flags base[0], base[stride], ... by a
-// whole wave once every one of them carries `want`; bounded like every wait.
+// Sum (or, with MAX, maximum) of the counts of `cnt` (< 65) flags base[0],
+// base[stride], ... by a whole wave once every one of them carries `want`;
+// bounded like every wait.
+template <bool MAX = false>
 __device__ __forceinline__ uint32_t wave_sum_flags(const uint64_t* base, int32_t stride,
                                                    int32_t cnt, uint32_t want,
                                                    bool* timed_out) {
@@ -232,7 +234,10 @@ __device__ __forceinline__ uint32_t wave_sum_flags(const uint64_t* base, int32_t
       continue;
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, kWave);
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint32_t x = __shfl_xor(c, o, kWave);
+      c = MAX ? max(c, x) : c + x;
+    }
     return c;
   }
   *timed_out = true;
