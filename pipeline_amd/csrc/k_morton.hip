@@ -50,29 +50,49 @@ __global__ __launch_bounds__(256) void morton_kernel(
     const int32_t cnt = min(T, seg_end(sv, r.seg) - base);
     const float mn = seg_min(sv, r.seg);
     const float rg = seg_range(sv, r.seg);
-#pragma unroll 4
-    for (int k = 0; k < PPT; ++k) {
-      const int32_t idx = k * 256 + tid;
-      if (idx < cnt) {
-        const int64_t g = static_cast<int64_t>(base) + idx;
-        float x, y, z;
+    auto emit = [&](int64_t g, float x, float y, float z) {
+      const uint32_t key = encode(x, y, z, mn, rg);
+      keys[g] = key;
+      if (HIST) {
+        atomicAdd(&s_hist[key & 255u], 1u);
+        atomicAdd(&s_hist[256 + ((key >> 8) & 255u)], 1u);
+        atomicAdd(&s_hist[512 + ((key >> 16) & 255u)], 1u);
+        atomicAdd(&s_hist[768 + (key >> 24)], 1u);
+      }
+    };
+    if (cnt == T) {
+      // Full tile: every point load is issued before the first one is used
+      // (a guarded load per iteration keeps one load in flight per wave).
+      float px[PPT], py[PPT], pz[PPT];
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) {
+        const int64_t g = static_cast<int64_t>(base) + k * 256 + tid;
         if (F3) {
-          x = pts[3 * g + 0];
-          y = pts[3 * g + 1];
-          z = pts[3 * g + 2];
+          px[k] = pts[3 * g + 0];
+          py[k] = pts[3 * g + 1];
+          pz[k] = pts[3 * g + 2];
         } else {
           const float4 p = reinterpret_cast<const float4*>(pts)[g];
-          x = p.x;
-          y = p.y;
-          z = p.z;
+          px[k] = p.x;
+          py[k] = p.y;
+          pz[k] = p.z;
         }
-        const uint32_t key = encode(x, y, z, mn, rg);
-        keys[g] = key;
-        if (HIST) {
-          atomicAdd(&s_hist[key & 255u], 1u);
-          atomicAdd(&s_hist[256 + ((key >> 8) & 255u)], 1u);
-          atomicAdd(&s_hist[512 + ((key >> 16) & 255u)], 1u);
-          atomicAdd(&s_hist[768 + (key >> 24)], 1u);
+      }
+#pragma unroll
+      for (int k = 0; k < PPT; ++k)
+        emit(static_cast<int64_t>(base) + k * 256 + tid, px[k], py[k], pz[k]);
+    } else {
+#pragma unroll 4
+      for (int k = 0; k < PPT; ++k) {
+        const int32_t idx = k * 256 + tid;
+        if (idx < cnt) {
+          const int64_t g = static_cast<int64_t>(base) + idx;
+          if (F3) {
+            emit(g, pts[3 * g + 0], pts[3 * g + 1], pts[3 * g + 2]);
+          } else {
+            const float4 p = reinterpret_cast<const float4*>(pts)[g];
+            emit(g, p.x, p.y, p.z);
+          }
         }
       }
     }
