@@ -46,20 +46,61 @@ __global__ __launch_bounds__(256) void edge_scan_kernel(
   const uint8_t* pn = prefix_all + sb;
   int32_t* edge = edge_all + sb;
 
+  // Loads first, all in flight together; then the edge counts.
+  int32_t la[IPT];
+  uint32_t lb[IPT];
+#pragma unroll
+  for (int k = 0; k < IPT; ++k) {
+    la[k] = 0;
+    lb[k] = 0;
+  }
+  // One loop per source, each a uniform choice, so the loads of a loop are
+  // issued together. The parent's prefix_n is stored beside the node by the
+  // radix tree in the fused build, else gathered through parent[] (never
+  // for node 0, whose edge count is 0).
+  const int32_t i0 = base + tid;
+  if (mode == 2) {
+#pragma unroll
+    for (int k = 0; k < IPT; ++k)
+      if (k * THREADS + tid < cnt) la[k] = edge[i0 + k * THREADS];
+  } else if (ppn_all && cnt == T) {
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+      la[k] = pn[i0 + k * THREADS];
+      lb[k] = ppn_all[sb + i0 + k * THREADS];
+    }
+  } else if (ppn_all) {
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+      if (k * THREADS + tid < cnt) {
+        la[k] = pn[i0 + k * THREADS];
+        lb[k] = ppn_all[sb + i0 + k * THREADS];
+      }
+    }
+  } else {
+    int32_t par[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+      par[k] = 0;
+      if (k * THREADS + tid < cnt) {
+        la[k] = pn[i0 + k * THREADS];
+        if (i0 + k * THREADS > 0) par[k] = par_all[sb + i0 + k * THREADS];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < IPT; ++k)
+      if (k * THREADS + tid < cnt && i0 + k * THREADS > 0) lb[k] = pn[par[k]];
+  }
   int32_t e[IPT];
 #pragma unroll
   for (int k = 0; k < IPT; ++k) {
     const int32_t i = base + k * THREADS + tid;
     int32_t v = 0;
     if (k * THREADS + tid < cnt) {
-      if (mode == 2) {
-        v = edge[i];
-      } else if (i > 0) {
-        // prefix_n of the parent: stored beside the node by the radix tree
-        // in the fused build, else gathered through parent[].
-        const uint32_t ppn = ppn_all ? ppn_all[sb + i] : pn[par_all[sb + i]];
-        v = static_cast<int32_t>(pn[i] / 3) - static_cast<int32_t>(ppn / 3);
-      }
+      if (mode == 2)
+        v = la[k];
+      else if (i > 0)
+        v = static_cast<int32_t>(static_cast<uint32_t>(la[k]) / 3) - static_cast<int32_t>(lb[k] / 3);
       if (mode != 2) edge[i] = v;
     }
     e[k] = v;

@@ -180,10 +180,14 @@ __global__ __launch_bounds__(256) void radix_tree_kernel(
     // every load of the window in flight at once
     constexpr int kRows = (kNodeTile + 2 * kHalo + 255) / 256;
     uint32_t rc[kRows];
+    const uint32_t* src = codes_all + sb + lo + static_cast<int32_t>(threadIdx.x);
+    if (hi - lo == kRows * 256) {  // a full window: no per-load guard
 #pragma unroll
-    for (int k = 0; k < kRows; ++k) {
-      const int32_t q = lo + static_cast<int32_t>(threadIdx.x) + k * 256;
-      if (q < hi) rc[k] = codes_all[sb + q];
+      for (int k = 0; k < kRows; ++k) rc[k] = src[k * 256];
+    } else {
+#pragma unroll
+      for (int k = 0; k < kRows; ++k)
+        if (lo + static_cast<int32_t>(threadIdx.x) + k * 256 < hi) rc[k] = src[k * 256];
     }
 #pragma unroll
     for (int k = 0; k < kRows; ++k) {

@@ -47,13 +47,19 @@ __global__ __launch_bounds__(256) void unique_kernel(
   const int32_t base = sb + r.local * T;
   const int32_t cnt = min(T, se - base);
 
+  // All loads are issued before any is used (a load guarded and consumed
+  // in the same iteration keeps one load in flight per wave).
   uint32_t keys[IPT];
+  const uint32_t* src = in + static_cast<int64_t>(base) + tid;
+  if (cnt == T) {
 #pragma unroll
-  for (int k = 0; k < IPT; ++k) {
-    const int32_t idx = k * THREADS + tid;
-    keys[k] = idx < cnt ? in[static_cast<int64_t>(base) + idx] : 0u;
-    s_keys[idx] = keys[k];
+    for (int k = 0; k < IPT; ++k) keys[k] = src[k * THREADS];
+  } else {
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) keys[k] = k * THREADS + tid < cnt ? src[k * THREADS] : 0u;
   }
+#pragma unroll
+  for (int k = 0; k < IPT; ++k) s_keys[k * THREADS + tid] = keys[k];
   const uint32_t before = (r.local > 0) ? in[static_cast<int64_t>(base) - 1] : 0u;
   __syncthreads();
 
