@@ -357,10 +357,16 @@ __global__ __launch_bounds__(WAVES * 64) void lsort_kernel(uint32_t* __restrict_
   }
   uint32_t kv[KPT];
   uint32_t mn = 0xffffffffu, mx = 0u;
+  // All loads first (consuming each in its own iteration keeps one load in
+  // flight per wave), then the bucket's range.
 #pragma unroll
   for (int k = 0; k < KPT; ++k) {
     const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
     kv[k] = idx < n ? g[idx] : 0u;
+  }
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
     if (idx < n) {
       mn = min(mn, kv[k]);
       mx = max(mx, kv[k]);
