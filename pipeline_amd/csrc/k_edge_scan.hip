@@ -39,10 +39,10 @@ __global__ __launch_bounds__(256) void edge_scan_kernel(
   const int32_t t = blockIdx.x;  // in-order dispatch: see lookback_n
   const TileRef r = find_tile(tile_start, sv.n_seg, t);
   const int32_t n = n_unique ? n_unique[r.seg] - 1 : n_brt_host;
+  const int32_t sb = seg_begin(sv, r.seg);  // loaded with n, before the test
   const int32_t base = r.local * T;
   if (base >= n) return;  // no later tile of this chain waits on it
   const int32_t cnt = min(T, n - base);
-  const int32_t sb = seg_begin(sv, r.seg);
   const uint8_t* pn = prefix_all + sb;
   int32_t* edge = edge_all + sb;
 

@@ -504,17 +504,27 @@ __global__ __launch_bounds__(kThreads) void oct_build_kernel(
   __shared__ int32_t s_count;
   __shared__ int32_t s_emax;
 
-  if (a.ctrl && a.ctrl[kCtrlOctOk] == 0) return;
   const TileRef r = find_tile(tile_start, sv.n_seg, xcd_tile(blockIdx.x, gridDim.x));
-  SegBind b;
-  b.m = a.n_unique ? a.n_unique[r.seg] - 1 : a.n_brt_host;
-  const int32_t i0 = r.local * kNodeTile;
-  // Nothing to do beyond the nodes; a single-key segment still has its root.
-  if (i0 >= b.m && !(i0 == 0 && b.m == 0)) return;
+  // The segment's scalars are loaded together before any is tested: one
+  // round trip instead of one per value.
+  // (The launcher guarantees these pointers; the build writes them all.)
+  const uint32_t fits = a.ctrl[kCtrlOctOk];
+  const int32_t m_seg = a.n_unique[r.seg] - 1;
+  const int32_t node_base = a.oct_start[r.seg];
+  const int32_t k_seg = a.kalloc[r.seg];
   const int32_t sb = seg_begin(sv, r.seg);
-  b.nodes = a.nodes + (a.oct_start ? a.oct_start[r.seg] : 0);
-  b.k = a.kalloc ? a.kalloc[r.seg] : a.k_host;
-  if (b.k < 1) return;
+  const float seg_mn = seg_min(sv, r.seg);
+  const float seg_rg = seg_range(sv, r.seg);
+  const int32_t i0 = r.local * kNodeTile;
+  // One test after all the loads (an early return per value would hold the
+  // loads after it back): the nodes fit, the tile has nodes (a single-key
+  // segment still has its root), the segment has slots and a valid base.
+  if (fits == 0 || (i0 >= m_seg && !(i0 == 0 && m_seg == 0)) || k_seg < 1 || node_base < 0)
+    return;
+  SegBind b;
+  b.m = m_seg;
+  b.nodes = a.nodes + node_base;
+  b.k = k_seg;
   b.codes = a.codes + sb;
   b.pn = a.prefix_n + sb;
   b.hl = a.has_leaf_left + sb;
@@ -522,8 +532,8 @@ __global__ __launch_bounds__(kThreads) void oct_build_kernel(
   b.lc = a.left_child + sb;
   b.e = a.edge + sb;
   b.p = a.offsets + sb;
-  b.mn = seg_min(sv, r.seg);
-  b.rg = seg_range(sv, r.seg);
+  b.mn = seg_mn;
+  b.rg = seg_rg;
 
   const int32_t lo = max(0, i0 - kHalo);
   const int32_t wn = max(0, min(b.m, i0 + kNodeTile + kHalo) - lo);
@@ -626,9 +636,11 @@ void launch_oct_build(const OctArgs& a, const SegView& sv,
                       hipStream_t s) {
   (void)mode;  // the fused build always writes every slot
   if (total_tiles <= 0) return;
-  // The group walk reads the children arrays.
+  // The group walk reads the children arrays; the per-segment counts, node
+  // bases and the fit flag come from the build.
   if (!a.has_leaf_left || !a.has_leaf_right || !a.left_child || !a.codes ||
-      !a.prefix_n || !a.edge || !a.offsets || !a.nodes)
+      !a.prefix_n || !a.edge || !a.offsets || !a.nodes || !a.ctrl || !a.n_unique ||
+      !a.oct_start || !a.kalloc)
     return;
   oct_build_kernel<<<total_tiles, kThreads, 0, s>>>(a, sv, tile_start);
 }

@@ -167,9 +167,9 @@ __global__ __launch_bounds__(256) void radix_tree_kernel(
   __shared__ uint32_t s_nwide;
   const TileRef r = find_tile(tile_start, sv.n_seg, xcd_tile(blockIdx.x, gridDim.x));
   const int32_t n = n_unique ? n_unique[r.seg] - 1 : n_brt_host;
+  const int32_t sb = seg_begin(sv, r.seg);  // loaded with n, before the test
   const int32_t i0 = r.local * kNodeTile;
   if (i0 >= n) return;
-  const int32_t sb = seg_begin(sv, r.seg);
   int32_t* parent = par_all + sb;
   uint8_t* parent_pn = ppn_all ? ppn_all + sb : nullptr;
   if (threadIdx.x == 0) s_nwide = 0;
