@@ -83,6 +83,10 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
   const int32_t sb = seg_begin(sv, r.seg);
   const int32_t base = sb + r.local * T;
   const int32_t cnt = min(T, seg_end(sv, r.seg) - base);
+  // The segment's count of this thread's digit, loaded now so that it is in
+  // hand when the tile's offsets are formed.
+  const uint32_t hv =
+      tid < 256 ? hist[static_cast<int64_t>(r.seg) * 1024 + hist_slot * 256 + tid] : 0u;
 
   uint32_t keys[KPT];
   uint32_t dg[KPT];
@@ -153,8 +157,6 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
   }
   uint32_t total;
   const uint32_t dstart = block_excl_scan<THREADS>(tile_cnt, s_tmp, &total);
-  const uint32_t hv =
-      tid < 256 ? hist[static_cast<int64_t>(r.seg) * 1024 + hist_slot * 256 + tid] : 0u;
   const uint32_t hex = block_excl_scan<THREADS>(hv, s_tmp, &total);
   if (tid < 256) {
     s_dstart[tid] = dstart;
@@ -179,10 +181,26 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
   }
   __syncthreads();
 
+  if (cnt == T) {
+    // All LDS reads first, then the stores (a guarded store per iteration
+    // waits on its two dependent LDS reads before the next begins).
+    uint32_t kv[KPT];
+    int32_t dst[KPT];
 #pragma unroll
-  for (int k = 0; k < KPT; ++k) {
-    const int32_t p = k * THREADS + tid;
-    if (p < cnt) out[s_dst[s_dig[p]] + p] = s_keys[p];
+    for (int k = 0; k < KPT; ++k) {
+      kv[k] = s_keys[k * THREADS + tid];
+      dst[k] = s_dig[k * THREADS + tid];
+    }
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) dst[k] = s_dst[dst[k]] + k * THREADS + tid;
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) out[dst[k]] = kv[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      const int32_t p = k * THREADS + tid;
+      if (p < cnt) out[s_dst[s_dig[p]] + p] = s_keys[p];
+    }
   }
 }
 
