@@ -132,11 +132,14 @@ __global__ __launch_bounds__(256) void edge_scan_kernel(
   const uint32_t excl = s_excl;
   int32_t* off = off_all + sb;
   int32_t kmax = 1;
+  uint32_t rb[IPT];  // all LDS reads before the stores
+#pragma unroll
+  for (int k = 0; k < IPT; ++k) rb[k] = s_rows[k * NW + w];
 #pragma unroll
   for (int k = 0; k < IPT; ++k) {
     const int32_t idx = k * THREADS + tid;
     if (idx < cnt) {
-      const int32_t p = static_cast<int32_t>(excl + s_rows[k * NW + w] + incl[k]);
+      const int32_t p = static_cast<int32_t>(excl + rb[k] + incl[k]);
       off[base + idx] = p;
       kmax = max(kmax, p + e[k]);
     }

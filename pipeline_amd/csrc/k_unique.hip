@@ -104,17 +104,26 @@ __global__ __launch_bounds__(256) void unique_kernel(
   const uint32_t excl = s_excl;
   const uint32_t total = s_total;
   const uint64_t lt = lanemask_lt();
+  // Every LDS read of a loop is issued before its results are used (a read
+  // consumed in its own iteration costs a round trip per iteration).
+  uint64_t mk[IPT];
+  uint32_t rb[IPT];
 #pragma unroll
   for (int k = 0; k < IPT; ++k) {
-    const uint64_t m = s_mask[k * NW + w];
-    if ((m >> lane) & 1ull) {
-      const uint32_t pos = s_rows[k * NW + w] + static_cast<uint32_t>(__popcll(m & lt));
-      s_keys[pos] = keys[k];
-    }
+    mk[k] = s_mask[k * NW + w];
+    rb[k] = s_rows[k * NW + w];
   }
+#pragma unroll
+  for (int k = 0; k < IPT; ++k)
+    if ((mk[k] >> lane) & 1ull) s_keys[rb[k] + static_cast<uint32_t>(__popcll(mk[k] & lt))] = keys[k];
   __syncthreads();
   uint32_t* dst = out + sb + excl;
-  for (uint32_t p = tid; p < total; p += THREADS) dst[p] = s_keys[p];
+  uint32_t kv[IPT];
+#pragma unroll
+  for (int k = 0; k < IPT; ++k) kv[k] = s_keys[k * THREADS + tid];
+#pragma unroll
+  for (int k = 0; k < IPT; ++k)
+    if (static_cast<uint32_t>(k * THREADS + tid) < total) dst[k * THREADS + tid] = kv[k];
 }
 
 template <int IPT>
