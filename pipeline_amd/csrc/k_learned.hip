@@ -148,6 +148,7 @@ __device__ __forceinline__ void load_cells(const LearnTables& tb, int tid, int t
 // scatter passes this measured faster than the cell table, whose 8 KB must
 // be loaded by every tile.
 struct CoarseDigit {
+  static constexpr bool kCheap = false;
   LearnTables tb;
   __device__ __forceinline__ void setup(int32_t, int tid, int threads) const {
     for (int i = tid; i < kLearnCoarse; i += threads) s_cb[i] = tb.fb[i * kLearnFine];
@@ -161,6 +162,7 @@ struct CoarseDigit {
   }
 };
 struct FineDigit {
+  static constexpr bool kCheap = false;
   LearnTables tb;
   __device__ __forceinline__ void setup(int32_t seg, int tid, int threads) const {
     for (int i = tid; i < kLearnFine; i += threads) s_fbs[i] = tb.fb[seg * kLearnFine + i];

@@ -14,9 +14,10 @@
 //      inclusive for a segment's first tile), then each of 256 threads looks
 //      back along its digit's chain, kOneSweepWindow tiles per round trip, with
 //      agent-scope loads, and publishes the inclusive prefix.
-//   4. Keys (and their digits) are scattered into LDS in digit order, then
-//      written out with consecutive threads on consecutive LDS slots, so
-//      equal-digit runs go to consecutive HBM addresses.
+//   4. Keys are scattered into LDS in digit order (with their digits, unless
+//      a digit is cheap to recompute), then written out with consecutive
+//      threads on consecutive LDS slots, so equal-digit runs go to
+//      consecutive HBM addresses.
 // Segments sort independently: each has its own digit histogram and its own
 // lookback chain, and no tile holds keys of two segments.
 //
@@ -44,6 +45,9 @@ constexpr int kOneSweepWindow = OCP_ONESWEEP_WINDOW;
 // digit is taken (per-segment tables go to LDS there); operator() maps a key
 // to its digit in [0, 256).
 struct RadixDigit {
+  // A digit this cheap is recomputed from the key in the scatter rather
+  // than staged in LDS beside it.
+  static constexpr bool kCheap = true;
   uint32_t shift;
   __device__ __forceinline__ void setup(int32_t, int, int) const {}
   __device__ __forceinline__ uint32_t operator()(uint32_t key) const {
@@ -63,10 +67,10 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
   constexpr int THREADS = WAVES * 64;
   constexpr int T = THREADS * KPT;
   static_assert(THREADS >= 256, "one thread per digit is required");
+  constexpr bool CHEAP = Digit::kCheap;
   __shared__ uint32_t s_keys[T];
-  __shared__ uint8_t s_dig[T];
+  __shared__ uint8_t s_dig[CHEAP ? 1 : T];
   __shared__ uint32_t s_whist[WAVES * 256];
-  __shared__ uint32_t s_dstart[256];
   __shared__ int32_t s_dst[256];
   __shared__ uint32_t s_tmp[WAVES];
 
@@ -159,7 +163,10 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
   const uint32_t dstart = block_excl_scan<THREADS>(tile_cnt, s_tmp, &total);
   const uint32_t hex = block_excl_scan<THREADS>(hv, s_tmp, &total);
   if (tid < 256) {
-    s_dstart[tid] = dstart;
+    // Each wave's offset of this digit becomes its LDS position in the
+    // tile: digit start + keys of the digit in earlier waves.
+#pragma unroll
+    for (int ww = 0; ww < WAVES; ++ww) s_whist[ww * 256 + tid] += dstart;
     uint32_t prev = 0;
     if (r.local > 0) {
       bool late = false;
@@ -174,10 +181,9 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
 
 #pragma unroll
   for (int k = 0; k < KPT; ++k) {
-    const uint32_t d = dg[k];
-    const uint32_t pos = s_dstart[d] + wh[d] + off[k];
+    const uint32_t pos = wh[dg[k]] + off[k];
     s_keys[pos] = keys[k];
-    s_dig[pos] = static_cast<uint8_t>(d);
+    if constexpr (!CHEAP) s_dig[pos] = static_cast<uint8_t>(dg[k]);
   }
   __syncthreads();
 
@@ -189,17 +195,24 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
 #pragma unroll
     for (int k = 0; k < KPT; ++k) {
       kv[k] = s_keys[k * THREADS + tid];
-      dst[k] = s_dig[k * THREADS + tid];
+      if constexpr (!CHEAP) dst[k] = s_dig[k * THREADS + tid];
     }
 #pragma unroll
-    for (int k = 0; k < KPT; ++k) dst[k] = s_dst[dst[k]] + k * THREADS + tid;
+    for (int k = 0; k < KPT; ++k) {
+      if constexpr (CHEAP) dst[k] = static_cast<int32_t>(dig(kv[k]));
+      dst[k] = s_dst[dst[k]] + k * THREADS + tid;
+    }
 #pragma unroll
     for (int k = 0; k < KPT; ++k) out[dst[k]] = kv[k];
   } else {
 #pragma unroll
     for (int k = 0; k < KPT; ++k) {
       const int32_t p = k * THREADS + tid;
-      if (p < cnt) out[s_dst[s_dig[p]] + p] = s_keys[p];
+      if (p < cnt) {
+        const uint32_t kv = s_keys[p];
+        const uint32_t d = CHEAP ? dig(kv) : s_dig[p];
+        out[s_dst[d] + p] = kv;
+      }
     }
   }
 }
