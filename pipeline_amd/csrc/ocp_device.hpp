@@ -121,6 +121,33 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* tmp,
   return off + incl - v;
 }
 
+// Two exclusive scans at once (one barrier pair): `tmp` holds 2 * THREADS/64
+// words of LDS. Ends with a barrier so `tmp` can be reused right away.
+template <int THREADS>
+__device__ __forceinline__ void block_excl_scan2(uint32_t a, uint32_t b, uint32_t* tmp,
+                                                 uint32_t* ea, uint32_t* eb) {
+  constexpr int NW = THREADS / kWave;
+  const uint32_t ia = wave_incl_scan(a);
+  const uint32_t ib = wave_incl_scan(b);
+  const uint32_t w = wave_id();
+  if (lane_id() == kWave - 1) {
+    tmp[w] = ia;
+    tmp[NW + w] = ib;
+  }
+  __syncthreads();
+  uint32_t oa = 0, ob = 0;
+#pragma unroll
+  for (int k = 0; k < NW; ++k) {
+    const uint32_t ta = tmp[k];
+    const uint32_t tb = tmp[NW + k];
+    oa += (static_cast<uint32_t>(k) < w) ? ta : 0u;
+    ob += (static_cast<uint32_t>(k) < w) ? tb : 0u;
+  }
+  __syncthreads();
+  *ea = oa + ia - a;
+  *eb = ob + ib - b;
+}
+
 // Exclusive scan, in place, of R words of LDS by one whole wave (each lane
 // takes R/64 consecutive words); returns the total in every lane.
 template <int R>

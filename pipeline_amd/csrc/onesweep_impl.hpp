@@ -72,7 +72,7 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
   __shared__ uint8_t s_dig[CHEAP ? 1 : T];
   __shared__ uint32_t s_whist[WAVES * 256];
   __shared__ int32_t s_dst[256];
-  __shared__ uint32_t s_tmp[WAVES];
+  __shared__ uint32_t s_tmp[2 * WAVES];
 
   const int32_t t = blockIdx.x;
   if (tiles_dev && t >= *tiles_dev) return;
@@ -159,9 +159,9 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
     flag_store(&flags[static_cast<int64_t>(t) * 256 + tid],
                flag_pack(tag, r.local == 0 ? kFlagInc : kFlagAgg, run));
   }
-  uint32_t total;
-  const uint32_t dstart = block_excl_scan<THREADS>(tile_cnt, s_tmp, &total);
-  const uint32_t hex = block_excl_scan<THREADS>(hv, s_tmp, &total);
+  // Tile-local digit starts and the segment's digit starts.
+  uint32_t dstart, hex;
+  block_excl_scan2<THREADS>(tile_cnt, hv, s_tmp, &dstart, &hex);
   if (tid < 256) {
     // Each wave's offset of this digit becomes its LDS position in the
     // tile: digit start + keys of the digit in earlier waves.
