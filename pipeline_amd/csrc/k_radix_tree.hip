@@ -50,6 +50,20 @@ struct Codes {
   __device__ __forceinline__ uint32_t local(int32_t q) const { return w[q - lo]; }
 };
 
+// delta(code_i, c) > dl as one subtraction and compare, for a fixed dl in
+// [0, 30]. With x = code_i ^ c: x == 0 gives delta 31 and x >= 2^31 gives
+// delta 255 (clz 0, minus one, as a byte), both above dl; otherwise delta =
+// clz(x) - 1 > dl exactly when x < 2^(30 - dl). So the predicate holds for
+// x < lim or x >= 2^31, i.e. for x - lim >= 2^31 - lim (unsigned).
+struct Above {
+  uint32_t ci;
+  uint32_t lim;
+  uint32_t thr;
+  __device__ __forceinline__ Above(uint32_t code_i, uint32_t dl)
+      : ci(code_i), lim(1u << (30u - dl)), thr(0x80000000u - (1u << (30u - dl))) {}
+  __device__ __forceinline__ bool operator()(uint32_t c) const { return ((c ^ ci) - lim) >= thr; }
+};
+
 #ifndef OCP_RADIX_HALO
 #define OCP_RADIX_HALO 512
 #endif
@@ -68,10 +82,10 @@ static_assert((kLocalMax & (kLocalMax - 1)) == 0 && kLocalLog >= 1 && kLocalLog 
 // ceil(l / divisor); with power-of-two divisors that is a shift, so no
 // integer division is left on the device.
 // Only called for local nodes (l < kLocalMax), whose probes stay in the window.
-__device__ __forceinline__ int32_t split_search(const Codes& codes, uint32_t code_i,
+template <bool CHECK>
+__device__ __forceinline__ int32_t split_search(const Codes& codes, const Above& above,
                                                 int32_t i, int32_t d,
-                                                int32_t l, int32_t n,
-                                                uint32_t delta_node) {
+                                                int32_t l, int32_t n) {
   int32_t s = 0;
   const int32_t lg = log2_ceil(static_cast<uint32_t>(l));
   const int32_t s_cutoff = (d == -1) ? i - 1 : n - i - 1;
@@ -80,8 +94,8 @@ __device__ __forceinline__ int32_t split_search(const Codes& codes, uint32_t cod
   for (int32_t k = 1; k <= kLocalLog; ++k) {
     const int32_t t = static_cast<int32_t>((static_cast<uint32_t>(l) + (1u << k) - 1u) >> k);
     const int32_t st = s + t;
-    const bool probe = k <= lg && st <= s_cutoff;
-    if (probe && delta(code_i, codes.local(probe ? i + st * d : i)) > delta_node) s = st;
+    const bool probe = k <= lg && (!CHECK || st <= s_cutoff);
+    if (probe && above(codes.local(probe ? i + st * d : i))) s = st;
   }
   return i + s * d + min(d, 0);
 }
@@ -154,6 +168,55 @@ __device__ __forceinline__ NodeOut wave_node(const Codes& codes, int32_t i, int3
 }
 
 
+// A node whose searches stay within kLocalMax of it, read from the LDS
+// window without the window test (the window spans kHalo >= kLocalMax + 2
+// around the tile). Returns false, emitting nothing, for a node the wave
+// path must finish: a far end beyond kLocalMax, or a threshold of 31 or
+// above (repeated codes, codes with bit 31 set), for which the wave path
+// evaluates delta itself. CHECK: bound the probes by [0, n] and the
+// reference's length and split cutoffs (tiles near either end).
+template <bool CHECK, class Emit>
+__device__ __forceinline__ bool local_node(const Codes& codes, int32_t i, int32_t n,
+                                           Emit& emit) {
+  const uint32_t code_i = codes.local(i);
+  const uint32_t dn = delta(code_i, codes.local(i + 1));
+  const uint32_t dp = delta(code_i, codes.local(i - 1));
+  const int32_t diff = static_cast<int32_t>(dn) - static_cast<int32_t>(dp);
+  const int32_t d = (diff > 0) - (diff < 0);
+  // delta(code_i, codes[i - d]); d == 0 (equal neighbours, repeated codes
+  // only) compares code i with itself: 31.
+  const uint32_t delta_min = d > 0 ? dp : (d < 0 ? dn : 31u);
+  if (delta_min >= 31u) return false;
+  const Above above_min(code_i, delta_min);
+  // Exponential search for an upper bound l_max (a power of two), then
+  // binary search of the length: t = l_max / 2, l_max / 4, ..., 1. The
+  // reference probes 2, 4, 8, ... until one fails; here the probes up to
+  // kLocalMax are read at once and l_max = 2^(leading passes + 1).
+  uint32_t pass = 0;
+#pragma unroll
+  for (int k = 1; k <= kLocalLog; ++k) {
+    const int32_t q = i + (1 << k) * d;
+    const bool in = !CHECK || (q >= 0 && q <= n);
+    const uint32_t cq = codes.local(in ? q : i);
+    pass |= static_cast<uint32_t>(in && above_min(cq)) << (k - 1);
+  }
+  const int32_t l_max = 2 << __builtin_ctz(~pass);
+  if (l_max > kLocalMax) return false;
+  const int32_t l_cutoff = (d == -1) ? i : n - i;
+  int32_t l = 0;
+#pragma unroll
+  for (int k = 1; k <= kLocalLog; ++k) {  // t = l_max / 2, l_max / 4, ..., 1
+    const int32_t t = l_max >> k;
+    const bool probe = t >= 1 && (!CHECK || l + t <= l_cutoff);
+    if (probe && above_min(codes.local(probe ? i + (l + t) * d : i))) l += t;
+  }
+  const int32_t j = i + l * d;
+  const uint32_t delta_node = delta(code_i, codes.local(j));
+  if (delta_node >= 31u) return false;
+  emit(i, delta_node, split_search<CHECK>(codes, Above(code_i, delta_node), i, d, l, n), j);
+  return true;
+}
+
 __global__ __launch_bounds__(256) void radix_tree_kernel(
     const uint32_t* __restrict__ codes_all, uint8_t* __restrict__ prefix_all,
     uint8_t* __restrict__ hl_all, uint8_t* __restrict__ hr_all,
@@ -215,6 +278,10 @@ __global__ __launch_bounds__(256) void radix_tree_kernel(
     }
   };
 
+  // A tile at least 2 * kLocalMax away from both ends of [0, n]: every
+  // probe of a local node (within kLocalMax + 2 of i) is inside [0, n] and
+  // below the length and split cutoffs, so those checks are skipped.
+  const bool interior = i0 >= 2 * kLocalMax && i0 + kNodeTile + 2 * kLocalMax <= n;
   for (int32_t k = 0; k < kNodeTile / 256; ++k) {
   const int32_t i = i0 + k * 256 + static_cast<int32_t>(threadIdx.x);
   bool wide = false;
@@ -222,46 +289,8 @@ __global__ __launch_bounds__(256) void radix_tree_kernel(
     if (i == 0) {
       wide = true;
     } else {
-      // Every probe of a local node lies within kLocalMax of i (and inside
-      // [0, n] by the bound checks), and the window spans kHalo >= kLocalMax
-      // around the tile, so these reads skip the window test.
-      const uint32_t code_i = codes.local(i);
-      const uint32_t dn = delta(code_i, codes.local(i + 1));
-      const uint32_t dp = delta(code_i, codes.local(i - 1));
-      const int32_t diff = static_cast<int32_t>(dn) - static_cast<int32_t>(dp);
-      const int32_t d = (diff > 0) - (diff < 0);
-      // delta(code_i, codes[i - d]); d == 0 (equal neighbours, repeated
-      // codes only) compares code i with itself.
-      const uint32_t delta_min = d > 0 ? dp : (d < 0 ? dn : delta(code_i, code_i));
-      // Exponential search for an upper bound l_max (a power of two), then
-      // binary search of the length: t = l_max / 2, l_max / 4, ..., 1. The
-      // reference probes 2, 4, 8, ... until one fails; here the probes up to
-      // kLocalMax are read at once and l_max = 2^(leading passes + 1).
-      uint32_t pass = 0;
-#pragma unroll
-      for (int k = 1; k <= kLocalLog; ++k) {
-        const int32_t q = i + (1 << k) * d;
-        const bool in = q >= 0 && q <= n;
-        const uint32_t cq = codes.local(in ? q : i);
-        pass |= static_cast<uint32_t>(in && delta(code_i, cq) > delta_min) << (k - 1);
-      }
-      const int32_t l_max = 2 << __builtin_ctz(~pass);
-      if (l_max > kLocalMax) {
-        wide = true;
-      } else {
-        const int32_t l_cutoff = (d == -1) ? i : n - i;
-        int32_t l = 0;
-#pragma unroll
-        for (int k = 1; k <= kLocalLog; ++k) {  // t = l_max / 2, l_max / 4, ..., 1
-          const int32_t t = l_max >> k;
-          const bool probe = t >= 1 && l + t <= l_cutoff;
-          if (probe && delta(code_i, codes.local(probe ? i + (l + t) * d : i)) > delta_min)
-            l += t;
-        }
-        const int32_t j = i + l * d;
-        const uint32_t delta_node = delta(code_i, codes.local(j));
-        emit(i, delta_node, split_search(codes, code_i, i, d, l, n, delta_node), j);
-      }
+      wide = interior ? !local_node<false>(codes, i, n, emit)
+                      : !local_node<true>(codes, i, n, emit);
     }
   }
   const uint64_t bal = __ballot(wide);
