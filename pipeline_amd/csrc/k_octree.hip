@@ -34,8 +34,8 @@
 //    it is 0; child_node_mask is the OR of the make-phase link bits.
 //
 // Layout of the work: a workgroup of 128 threads takes 256 consecutive BRT
-// nodes and stages a packed 16-byte record {P, e, left child + leaf flags,
-// prefix length} and the code of those nodes plus 32 on either side into
+// nodes and stages two packed 8-byte records {P, e + prefix length} and
+// {left child + leaf flags, code} of those nodes plus 32 on either side into
 // LDS with coalesced loads. It compacts the non-empty chains of its range
 // so that every lane processes a chain. The short walks over neighbouring
 // chains and most child lookups of the group walk are LDS reads (anything
@@ -71,6 +71,22 @@ constexpr uint32_t kChildMask = (1u << 30) - 1u;
 
 typedef int32_t v4i __attribute__((ext_vector_type(4)));
 typedef int32_t v3i __attribute__((ext_vector_type(3)));
+typedef int32_t v2i __attribute__((ext_vector_type(2)));
+
+// A window node is two 8-byte LDS records: {P, e | prefix length << 16} and
+// {left child | leaf flags, code}. Edge counts are small signed values (at
+// most 85 in magnitude for byte prefix lengths) and prefix lengths bytes, so
+// both fit one word, and 16 bytes per node (20 with a separate code array)
+// keep the workgroup's LDS within what 8 waves per SIMD allow.
+__device__ __forceinline__ int32_t pack_epn(int32_t e, uint32_t pn) {
+  return static_cast<int32_t>((static_cast<uint32_t>(e) & 0xffffu) | (pn << 16));
+}
+__device__ __forceinline__ int32_t epn_e(int32_t w) {
+  return static_cast<int32_t>(static_cast<int16_t>(static_cast<uint32_t>(w) & 0xffffu));
+}
+__device__ __forceinline__ int32_t epn_pn(int32_t w) {
+  return static_cast<int32_t>(static_cast<uint32_t>(w) >> 16);
+}
 
 struct SegBind {
   OctNode* nodes;
@@ -102,34 +118,37 @@ struct View {
   int32_t lo;       // window [lo, lo + n) of nodes, [lo, lo + nc) of codes
   int32_t n;
   int32_t nc;
-  lds_ptr<const v4i> rec;
-  lds_ptr<const uint32_t> code;
+  lds_ptr<const v2i> pe;   // {P, e | pn << 16}
+  lds_ptr<const v2i> lcc;  // {left child | flags, code}
   lds_ptr<uint32_t> sc_col;  // this thread's SlotChildren column
   __device__ __forceinline__ Rec R(int32_t q) const {
     if (static_cast<uint32_t>(q - lo) < static_cast<uint32_t>(n)) {
-      const v4i r = rec[q - lo];
-      return {r.x, r.y, static_cast<uint32_t>(r.z), r.w};
+      const v2i a = pe[q - lo];
+      const v2i c = lcc[q - lo];
+      return {a.x, epn_e(a.y), static_cast<uint32_t>(c.x), epn_pn(a.y)};
     }
     const uint32_t l = static_cast<uint32_t>(b.lc[q]) | (b.hl[q] ? kLeafL : 0u) |
                        (b.hr[q] ? kLeafR : 0u);
     return {b.p[q], b.e[q], l, static_cast<int32_t>(b.pn[q])};
   }
   __device__ __forceinline__ int32_t P(int32_t q) const {
-    return static_cast<uint32_t>(q - lo) < static_cast<uint32_t>(n) ? rec[q - lo].x : b.p[q];
+    return static_cast<uint32_t>(q - lo) < static_cast<uint32_t>(n) ? pe[q - lo].x : b.p[q];
   }
   __device__ __forceinline__ int32_t E(int32_t q) const {
-    return static_cast<uint32_t>(q - lo) < static_cast<uint32_t>(n) ? rec[q - lo].y : b.e[q];
+    return static_cast<uint32_t>(q - lo) < static_cast<uint32_t>(n) ? epn_e(pe[q - lo].y) : b.e[q];
   }
   __device__ __forceinline__ uint32_t CODE(int32_t q) const {
-    return static_cast<uint32_t>(q - lo) < static_cast<uint32_t>(nc) ? code[q - lo] : b.codes[q];
+    return static_cast<uint32_t>(q - lo) < static_cast<uint32_t>(nc)
+               ? static_cast<uint32_t>(lcc[q - lo].y) : b.codes[q];
   }
   // Record and code of internal node q (q < m) under one window test: the
   // code window is at least as long as the node window.
   __device__ __forceinline__ Rec RC(int32_t q, uint32_t* c) const {
     if (static_cast<uint32_t>(q - lo) < static_cast<uint32_t>(n)) {
-      const v4i r = rec[q - lo];
-      *c = code[q - lo];
-      return {r.x, r.y, static_cast<uint32_t>(r.z), r.w};
+      const v2i a = pe[q - lo];
+      const v2i l = lcc[q - lo];
+      *c = static_cast<uint32_t>(l.y);
+      return {a.x, epn_e(a.y), static_cast<uint32_t>(l.x), epn_pn(a.y)};
     }
     *c = b.codes[q];
     const uint32_t l = static_cast<uint32_t>(b.lc[q]) | (b.hl[q] ? kLeafL : 0u) |
@@ -403,8 +422,8 @@ __device__ __forceinline__ void build_chain(const View& v, int32_t i, int32_t ro
     int32_t q = i + 1;
     bool ended = false;
     for (const int32_t q_win = min(b.m, v.lo + v.n); q < q_win; ++q) {
-      const v4i r = v.rec[q - v.lo];
-      if (fwd(r.x, r.y)) {
+      const v2i r = v.pe[q - v.lo];
+      if (fwd(r.x, epn_e(r.y))) {
         ended = true;
         break;
       }
@@ -444,17 +463,17 @@ __device__ __forceinline__ void build_chain(const View& v, int32_t i, int32_t ro
     int32_t q = i - 1;
     bool ended = false;
     for (const int32_t q_win = max(1, v.lo); q >= q_win; --q) {  // inside the window
-      const v4i r = v.rec[q - v.lo];
+      const v2i r = v.pe[q - v.lo];
       const int32_t pq = r.x;
       if (pq + emax_win <= p_i && !before_matters) {
         ended = true;
         break;
       }
-      const int32_t eq = min(r.y, kMaxEdge);
+      const int32_t eq = min(epn_e(r.y), kMaxEdge);
       if (eq <= 0) continue;
       reach = max(reach, pq + eq);
       if (pq + eq - 1 < p_i) continue;
-      back_links(q, pq, eq, v.code[q - v.lo], r.w / 3);
+      back_links(q, pq, eq, static_cast<uint32_t>(v.lcc[q - v.lo].y), epn_pn(r.y) / 3);
     }
     if (!ended) {
       for (; q >= 1; --q) {  // before the window
@@ -494,10 +513,14 @@ __device__ __forceinline__ void build_chain(const View& v, int32_t i, int32_t ro
   store_zeros(b.nodes, max(reach, 1), min(next_start, b.k));
 }
 
-__global__ __launch_bounds__(kThreads) void oct_build_kernel(
+// Eight waves per SIMD: left to itself the compiler gives the kernel 106
+// scalar registers, which caps it at seven (800 per SIMD); asked for eight
+// it keeps 78 and parks the rest in VGPR lanes. At 10M uniform points:
+// 276 -> 254 us together with the 16-byte window records.
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void oct_build_kernel(
     OctArgs a, SegView sv, const int32_t* __restrict__ tile_start) {
-  __shared__ v4i s_rec[kWinN];
-  __shared__ uint32_t s_code[kWinN];
+  __shared__ v2i s_pe[kWinN];
+  __shared__ v2i s_lcc[kWinN];
   static_assert(kNodeTile <= 256, "work items are stored as bytes");
   __shared__ uint8_t s_list[kNodeTile];  // work items as i - i0
   __shared__ uint32_t s_sc[8 * kThreads];
@@ -568,18 +591,20 @@ __global__ __launch_bounds__(kThreads) void oct_build_kernel(
 #pragma unroll
   for (int k = 0; k < kRows; ++k) {
     const int32_t q = tid + k * kThreads;
-    if (q < wnc) s_code[q] = r_code[k];
     if (q < wn) {
       const uint32_t l = static_cast<uint32_t>(r_lc[k]) | (r_hl[k] ? kLeafL : 0u) |
                          (r_hr[k] ? kLeafR : 0u);
-      s_rec[q] = v4i{r_p[k], r_e[k], static_cast<int32_t>(l), static_cast<int32_t>(r_pn[k])};
+      s_pe[q] = v2i{r_p[k], pack_epn(r_e[k], r_pn[k])};
+      s_lcc[q] = v2i{static_cast<int32_t>(l), static_cast<int32_t>(r_code[k])};
       emax = max(emax, r_e[k]);
+    } else if (q < wnc) {
+      s_lcc[q] = v2i{0, static_cast<int32_t>(r_code[k])};
     }
   }
   emax = wave_incl_max(emax);
   if (lane_id() == 0) atomicMax(&s_emax, emax);
   __syncthreads();
-  const View v{b, lo, wn, wnc, as_lds<const v4i>(s_rec), as_lds<const uint32_t>(s_code),
+  const View v{b, lo, wn, wnc, as_lds<const v2i>(s_pe), as_lds<const v2i>(s_lcc),
                as_lds<uint32_t>(s_sc + tid)};
   // Compact the block's work: the root and the non-empty chains.
 #pragma unroll
