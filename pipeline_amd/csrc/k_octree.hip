@@ -60,9 +60,12 @@ constexpr int kThreads = 128;
 #ifndef OCP_OCT_HALO
 #define OCP_OCT_HALO 32
 #endif
-// BRT nodes staged on either side of a tile. LDS per workgroup bounds the
-// waves per CU here: 10M uniform points, halo 8 -> 330 us, 16 -> 308,
-// 32 -> 286, 64 -> 296, 128 -> 340, 256 -> 422.
+// BRT nodes staged on either side of a tile. With 16-byte window records
+// and a 32-bit last-writer table: 10M uniform points, halo 8 -> 330 us,
+// 16 -> 308, 32 -> 286, 64 -> 296, 128 -> 340, 256 -> 422 (LDS per
+// workgroup bounded the waves per CU). At 8 waves per SIMD with 8-byte
+// records, halos up to 48 keep that occupancy and time the same
+// (32 -> 256 us, 40 -> 258, 48 -> 254).
 constexpr int kHalo = OCP_OCT_HALO;
 constexpr int kWinN = kNodeTile + 2 * kHalo;
 constexpr uint32_t kLeafL = 1u << 30;
