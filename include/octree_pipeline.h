@@ -56,7 +56,23 @@ typedef struct ocp_oct_node {
 int ocp_create(ocp_ctx* out, int device, void* stream);
 int ocp_destroy(ocp_ctx ctx);
 const char* ocp_last_error(ocp_ctx ctx);
+/* Waits for the context's stream. Returns OCP_ERR_STATE when a lookback
+ * wait of the work enqueued since the last check gave up (see
+ * ocp_set_wait_limit): that work's output is invalid, the pending build
+ * result is dropped, and the condition is cleared, so the next call runs
+ * normally. ocp_get_results, ocp_get_counts and the synchronous entries
+ * report it the same way. The reference checks with checkCudaErrors and
+ * exits instead (include/cuda/common/helper_cuda.hpp:583-602). */
 int ocp_synchronize(ocp_ctx ctx);
+/* Every wait for an earlier tile (the decoupled lookbacks of the sort,
+ * unique and scan stages) gives up after `polls` unsuccessful polls
+ * (polls < 0: the default, 2^22). Once one has given up, every later
+ * lookback-stage kernel on the context does nothing until the condition is
+ * reported. With polls = 0 every wait gives up at once and every kernel
+ * still runs to its end: a way to exercise the error path. No counterpart
+ * in the reference, whose waits spin unbounded
+ * (kernels/cuda/02_sort.cu:688-715, 735-755). */
+int ocp_set_wait_limit(ocp_ctx ctx, int64_t polls);
 /* Plumbing for callers without their own allocator (tests, bindings). */
 int ocp_malloc(ocp_ctx ctx, void** ptr, size_t bytes);
 int ocp_free(ocp_ctx ctx, void* ptr);

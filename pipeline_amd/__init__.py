@@ -89,6 +89,7 @@ SIGNATURES = {
     "ocp_destroy": (_INT, [_P]),
     "ocp_last_error": (ctypes.c_char_p, [_P]),
     "ocp_synchronize": (_INT, [_P]),
+    "ocp_set_wait_limit": (_INT, [_P, _I64]),
     "ocp_malloc": (_INT, [_P, ctypes.POINTER(_P), _SZ]),
     "ocp_free": (_INT, [_P, _P]),
     "ocp_memcpy_h2d": (_INT, [_P, _P, _P, _SZ]),
@@ -271,6 +272,10 @@ class Context:
 
     def synchronize(self):
         self._check(self.lib.ocp_synchronize(self.h))
+
+    def set_wait_limit(self, polls: int):
+        """Polls after which a lookback wait gives up (-1: the default)."""
+        self._check(self.lib.ocp_set_wait_limit(self.h, int(polls)))
 
     def copy(self, dst: DeviceArray, src: DeviceArray, nbytes: int):
         """Device-to-device copy kernel (16-byte multiples)."""

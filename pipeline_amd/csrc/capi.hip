@@ -417,24 +417,26 @@ OctArgs pipeline_oct_args(ocp_ctx c) {
   return a;
 }
 
-void enqueue_octree(ocp_ctx c, const Plan& p, bool capturing) {
+// False when the octree kernel refused its arguments (nothing launched).
+bool enqueue_octree(ocp_ctx c, const Plan& p, bool capturing) {
   Workspace& w = c->ws;
   hipStream_t s = c->stream;
   int32_t* kalloc = w.counts + w.seg_cap;
   { KScope k(c, kKOctOffsets, capturing);
     launch_oct_offsets(kalloc, p.n_seg, w.oct_start, c->ctrl, w.k_cap, s); }
   const OctArgs a = pipeline_oct_args(c);
-  { KScope k(c, kKOctBuild, capturing);
-    launch_oct_build(a, p.sv, p.ts_node, p.tiles_node,
-                     kOctMake | kOctLink | kOctRoot | kOctFull, s); }
+  KScope k(c, kKOctBuild, capturing);
+  return launch_oct_build(a, p.sv, p.ts_node, p.tiles_node,
+                          kOctMake | kOctLink | kOctRoot | kOctFull, s);
 }
 
 void mark(ocp_ctx c, int k, bool capturing) {
   if (c->stage_timing && !capturing) (void)hipEventRecord(c->ev[k], c->stream);
 }
 
-// The whole chain, enqueued on the context stream.
-void enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
+// The whole chain, enqueued on the context stream. False when a launcher
+// refused its arguments.
+bool enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
                       bool capturing) {
   Workspace& w = c->ws;
   hipStream_t s = c->stream;
@@ -466,14 +468,15 @@ void enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
   mark(c, 3, capturing);
   { KScope k(c, kKRadixTree, capturing);
     launch_radix_tree(w.keys_b, w.prefix_n, w.hl, w.hr, w.lc, w.par, w.ppn, p.sv, p.ts_node,
-                      p.tiles_node, n_unique, 0, s); }
+                      p.tiles_node, n_unique, 0, c->ctrl, s); }
   mark(c, 4, capturing);
   { KScope k(c, kKEdgeScan, capturing);
     launch_edge_scan(w.prefix_n, w.par, w.ppn, w.edge, w.offs, p.sv, p.ts_scan, p.tiles_scan,
                      n_unique, 0, w.flags, c->ctrl, kalloc, 0, tt, s); }
   mark(c, 5, capturing);
-  enqueue_octree(c, p, capturing);
+  const bool ok = enqueue_octree(c, p, capturing);
   mark(c, 6, capturing);
+  return ok;
 }
 
 std::string graph_key(const char* kind, const float* pts, int f3, int64_t n,
@@ -499,13 +502,17 @@ int run_plan(ocp_ctx c, const float* pts, int f3, const Plan& p,
     if (it == c->graphs.end()) {
       hipGraph_t g = nullptr;
       OCP_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      enqueue_pipeline(c, pts, f3 != 0, p, true);
+      const bool ok = enqueue_pipeline(c, pts, f3 != 0, p, true);
       hipError_t e = hipGetLastError();
       hipError_t e2 = hipStreamEndCapture(c->stream, &g);
       if (e != hipSuccess || e2 != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
         return fail(c, OCP_ERR_HIP, std::string("graph capture: ") +
                                         hipGetErrorString(e != hipSuccess ? e : e2));
+      }
+      if (!ok) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(c, OCP_ERR_INVALID, "octree stage: a stage array is missing");
       }
       hipGraphExec_t ex = nullptr;
       e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
@@ -516,8 +523,9 @@ int run_plan(ocp_ctx c, const float* pts, int f3, const Plan& p,
     OCP_HIP(c, hipGraphLaunch(it->second, c->stream));
     c->ev_recorded = false;
   } else {
-    enqueue_pipeline(c, pts, f3 != 0, p, false);
+    const bool ok = enqueue_pipeline(c, pts, f3 != 0, p, false);
     OCP_CHECK_LAUNCH(c);
+    if (!ok) return fail(c, OCP_ERR_INVALID, "octree stage: a stage array is missing");
     c->ev_recorded = c->stage_timing;
   }
   c->last_plan = p;
@@ -525,23 +533,45 @@ int run_plan(ocp_ctx c, const float* pts, int f3, const Plan& p,
   return OCP_OK;
 }
 
-// After a build: if the octree did not fit, grow the node buffer and redo
-// stage 7 (its inputs are intact). Synchronises.
-int settle(ocp_ctx c) {
-  if (!c->have_result) return fail(c, OCP_ERR_STATE, "no build has run");
+// A lookback wait that gave up since the last check: the word is cleared
+// (the next launch runs normally), the pending build result is dropped and
+// the error is returned. `hc` holds the control block read after the
+// stream was synchronised.
+int report_timeout(ocp_ctx c, const uint32_t* hc) {
+  if (hc[kCtrlTimeout] == 0) return OCP_OK;
+  c->have_result = false;
+  launch_fill_u32(c->ctrl + kCtrlTimeout, 0u, 1, c->stream);
+  OCP_CHECK_LAUNCH(c);
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  return fail(c, OCP_ERR_STATE,
+              "a lookback wait gave up (wait limit reached); the result of the work "
+              "enqueued since the last check is invalid");
+}
+
+// Synchronises the stream and reports a wait that gave up.
+int sync_and_check(ocp_ctx c) {
   uint32_t hc[kCtrlWords];
   OCP_HIP(c, hipMemcpyAsync(hc, c->ctrl, sizeof(hc), hipMemcpyDeviceToHost, c->stream));
   OCP_HIP(c, hipStreamSynchronize(c->stream));
-  if (hc[kCtrlTimeout] != 0) {
-    OCP_HIP(c, hipMemsetAsync(c->ctrl + kCtrlTimeout, 0, sizeof(uint32_t), c->stream));
-    OCP_HIP(c, hipStreamSynchronize(c->stream));
-    return fail(c, OCP_ERR_HIP, "a lookback wait hit its spin limit; the result is invalid");
-  }
+  return report_timeout(c, hc);
+}
+
+// After a build: if the octree did not fit, grow the node buffer and redo
+// stage 7 (its inputs are intact). Synchronises.
+int settle(ocp_ctx c) {
+  if (!c->have_result) return fail(c, OCP_ERR_STATE, "no build result (none has run, or it failed)");
+  uint32_t hc[kCtrlWords];
+  OCP_HIP(c, hipMemcpyAsync(hc, c->ctrl, sizeof(hc), hipMemcpyDeviceToHost, c->stream));
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  if (int r = report_timeout(c, hc)) return r;
   if (hc[kCtrlOctOk] == 0) {
     const int64_t need = hc[kCtrlOctTotal];
     clear_graphs(c);
     if (int r = alloc_nodes(c, need + need / 4 + 1024)) return r;
-    enqueue_octree(c, c->last_plan, true);
+    if (!enqueue_octree(c, c->last_plan, true)) {
+      c->have_result = false;
+      return fail(c, OCP_ERR_INVALID, "octree stage: a stage array is missing");
+    }
     OCP_CHECK_LAUNCH(c);
     OCP_HIP(c, hipMemcpyAsync(hc, c->ctrl, sizeof(hc), hipMemcpyDeviceToHost, c->stream));
     OCP_HIP(c, hipStreamSynchronize(c->stream));
@@ -623,7 +653,15 @@ const char* ocp_last_error(ocp_ctx c) { return c ? c->err.c_str() : "null contex
 
 int ocp_synchronize(ocp_ctx c) {
   if (check_ctx(c)) return OCP_ERR_INVALID;
-  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  return sync_and_check(c);
+}
+
+int ocp_set_wait_limit(ocp_ctx c, int64_t polls) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (polls > int64_t(0xfffffffe)) return fail(c, OCP_ERR_INVALID, "polls > 2^32-2");
+  const uint32_t v = polls < 0 ? 0u : static_cast<uint32_t>(polls) + 1u;
+  launch_fill_u32(c->ctrl + kCtrlWaitLimit, v, 1, c->stream);
+  OCP_CHECK_LAUNCH(c);
   return OCP_OK;
 }
 
@@ -774,7 +812,8 @@ int ocp_k_BuildRadixTree(ocp_ctx c, int32_t n, const uint32_t* codes, uint8_t* p
     return fail(c, OCP_ERR_INVALID, "null pointer");
   if (n > kMaxPoints) return fail(c, OCP_ERR_CAPACITY, "n > 2^30-1");
   launch_radix_tree(codes, prefix_n, hl, hr, lc, parent, nullptr, single_view(n + 1, 0.0f, 1.0f),
-                    nullptr, static_cast<int32_t>(cdiv(n, kNodeTile)), nullptr, n, c->stream);
+                    nullptr, static_cast<int32_t>(cdiv(n, kNodeTile)), nullptr, n, c->ctrl,
+                    c->stream);
   OCP_CHECK_LAUNCH(c);
   return OCP_OK;
 }
@@ -1226,6 +1265,7 @@ int ocp_cdf_train(ocp_ctx c, int32_t steps, float lr, float momentum, uint32_t s
   OCP_HIP(c, hipMemcpyAsync(L.samp_a, L.ring, sizeof(uint32_t) * ns, hipMemcpyDeviceToDevice,
                             c->stream));
   if (int r = ocp_k_SortKeys(c, L.samp_a, L.samp_b, static_cast<uint32_t>(ns))) return r;
+  if (int r = sync_and_check(c)) return r;  // the sorted samples must be valid
   launch_lmeta(L.samp_a, ns, L.meta, c->stream);
   launch_llinear(L.model, c->stream);
   launch_ltrain_root(L.samp_a, ns, L.meta, steps, lr, momentum, seed, L.model, L.loss, c->stream);
@@ -1412,10 +1452,7 @@ int ocp_sort_learned(ocp_ctx c, uint32_t* keys, uint32_t* alt, uint32_t n,
   uint32_t cw[kCtrlWords];
   OCP_HIP(c, hipMemcpyAsync(cw, c->ctrl, sizeof(cw), hipMemcpyDeviceToHost, c->stream));
   OCP_HIP(c, hipStreamSynchronize(c->stream));
-  if (cw[kCtrlTimeout]) {
-    OCP_HIP(c, hipMemsetAsync(c->ctrl + kCtrlTimeout, 0, sizeof(uint32_t), c->stream));
-    return fail(c, OCP_ERR_STATE, "a lookback wait hit its spin limit; the result is invalid");
-  }
+  if (int r = report_timeout(c, cw)) return r;
   local.misordered = static_cast<int32_t>(cw[kCtrlLearnBad]);
   if (local.misordered > 0) {
     if (int r = ocp_k_SortKeys(c, keys, alt, n)) return r;

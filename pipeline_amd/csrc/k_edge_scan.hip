@@ -37,6 +37,8 @@ __global__ __launch_bounds__(256) void edge_scan_kernel(
   const uint32_t lane = lane_id();
   const bool scan = mode != 1;
   const int32_t t = blockIdx.x;  // in-order dispatch: see lookback_n
+  const uint32_t tag = ctrl[kCtrlEpoch] * 8u + kStageScan;
+  if (stopped(ctrl, tag)) return;
   const TileRef r = find_tile(tile_start, sv.n_seg, t);
   const int32_t n = n_unique ? n_unique[r.seg] - 1 : n_brt_host;
   const int32_t sb = seg_begin(sv, r.seg);  // loaded with n, before the test
@@ -116,15 +118,15 @@ __global__ __launch_bounds__(256) void edge_scan_kernel(
   __syncthreads();
   if (w == 0) {
     const uint32_t total = wave_scan_lds<IPT * NW>(s_rows);
-    const uint32_t tag = ctrl[kCtrlEpoch] * 8u + kStageScan;
     if (lane == 0)
       flag_store(&flags[t], flag_pack(tag, kFlagAgg, total));
     bool late = false;
     // Group sums live after the grid's tile flags.
     const uint32_t prev =
-        wave_lookback_groups(flags, flags + gridDim.x, t, t - r.local, total, tag, &late);
+        wave_lookback_groups(flags, flags + gridDim.x, t, t - r.local, total, tag,
+                             wait_limit(ctrl), &late);
     if (lane == 0) {
-      if (late) ctrl[kCtrlTimeout] = 1u;
+      if (late) raise_timeout(ctrl, tag);
       s_excl = prev;
     }
   }
@@ -157,7 +159,6 @@ __global__ __launch_bounds__(256) void edge_scan_kernel(
       // earlier tiles published, so the segment's counter takes one atomic
       // per group (same-address atomics from every tile serialise: 19 us at
       // 10M). The waits go to earlier tiles only.
-      const uint32_t tag = ctrl[kCtrlEpoch] * 8u + kStageScan;
       uint64_t* mflags = flags + 2 * static_cast<int64_t>(gridDim.x);
       if (lane == 0) flag_store(mflags + t, flag_pack(tag, kFlagAgg, static_cast<uint32_t>(m)));
       const int32_t local = r.local;
@@ -165,9 +166,9 @@ __global__ __launch_bounds__(256) void edge_scan_kernel(
       if ((local & 63) == 63 || local == (n + T - 1) / T - 1) {
         bool late = false;
         const uint32_t gm = wave_sum_flags<true>(mflags + (t - local) + g0, 1, local - g0,
-                                                 tag & 0x3fffffffu, &late);
+                                                 tag & 0x3fffffffu, wait_limit(ctrl), &late);
         if (lane == 0) {
-          if (late) ctrl[kCtrlTimeout] = 1u;
+          if (late) raise_timeout(ctrl, tag);
           atomicMax(&kalloc[r.seg], max(m, static_cast<int32_t>(gm)));
         }
       }

@@ -545,7 +545,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
   // One test after all the loads (an early return per value would hold the
   // loads after it back): the nodes fit, the tile has nodes (a single-key
   // segment still has its root), the segment has slots and a valid base.
-  if (fits == 0 || (i0 >= m_seg && !(i0 == 0 && m_seg == 0)) || k_seg < 1 || node_base < 0)
+  // After a lookback time-out the inputs are not valid.
+  const uint32_t stop = a.ctrl[kCtrlTimeout];
+  if (fits == 0 || stop != 0 || (i0 >= m_seg && !(i0 == 0 && m_seg == 0)) || k_seg < 1 ||
+      node_base < 0)
     return;
   SegBind b;
   b.m = m_seg;
@@ -659,18 +662,18 @@ void launch_oct_root(OctNode* nodes, const uint32_t* codes,
   oct_root_kernel<<<1, 64, 0, s>>>(nodes, codes, prefix_n, n_brt, mn, rg);
 }
 
-void launch_oct_build(const OctArgs& a, const SegView& sv,
+bool launch_oct_build(const OctArgs& a, const SegView& sv,
                       const int32_t* tile_start, int32_t total_tiles, int mode,
                       hipStream_t s) {
   (void)mode;  // the fused build always writes every slot
-  if (total_tiles <= 0) return;
   // The group walk reads the children arrays; the per-segment counts, node
   // bases and the fit flag come from the build.
   if (!a.has_leaf_left || !a.has_leaf_right || !a.left_child || !a.codes ||
       !a.prefix_n || !a.edge || !a.offsets || !a.nodes || !a.ctrl || !a.n_unique ||
       !a.oct_start || !a.kalloc)
-    return;
-  oct_build_kernel<<<total_tiles, kThreads, 0, s>>>(a, sv, tile_start);
+    return false;
+  if (total_tiles > 0) oct_build_kernel<<<total_tiles, kThreads, 0, s>>>(a, sv, tile_start);
+  return true;
 }
 
 }  // namespace ocp

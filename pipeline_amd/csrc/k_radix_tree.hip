@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void radix_tree_kernel(
     int32_t* __restrict__ lc_all, int32_t* __restrict__ par_all,
     uint8_t* __restrict__ ppn_all, SegView sv,
     const int32_t* __restrict__ tile_start, const int32_t* __restrict__ n_unique,
-    int32_t n_brt_host) {
+    int32_t n_brt_host, const uint32_t* __restrict__ ctrl) {
   static_assert(kLocalMax <= kHalo / 2 + 1, "local searches must stay in the window");
   __shared__ uint32_t s_codes[kNodeTile + 2 * kHalo];
   __shared__ int32_t s_wide[kNodeTile];
@@ -232,7 +232,8 @@ __global__ __launch_bounds__(256) void radix_tree_kernel(
   const int32_t n = n_unique ? n_unique[r.seg] - 1 : n_brt_host;
   const int32_t sb = seg_begin(sv, r.seg);  // loaded with n, before the test
   const int32_t i0 = r.local * kNodeTile;
-  if (i0 >= n) return;
+  // After a lookback time-out the unique keys are not valid input.
+  if (i0 >= n || (ctrl && stopped_any(ctrl))) return;
   int32_t* parent = par_all + sb;
   uint8_t* parent_pn = ppn_all ? ppn_all + sb : nullptr;
   if (threadIdx.x == 0) s_nwide = 0;
@@ -336,12 +337,12 @@ void launch_radix_tree(const uint32_t* codes, uint8_t* prefix_n,
                        int32_t* left_child, int32_t* parent, uint8_t* parent_pn,
                        const SegView& sv, const int32_t* tile_start, int32_t total_tiles,
                        const int32_t* n_unique, int32_t n_brt_host,
-                       hipStream_t s) {
+                       const uint32_t* ctrl, hipStream_t s) {
   if (total_tiles <= 0) return;
   static_assert(kNodeTile % 256 == 0, "whole node rows per thread");
   radix_tree_kernel<<<total_tiles, 256, 0, s>>>(
       codes, prefix_n, has_leaf_left, has_leaf_right, left_child, parent, parent_pn, sv,
-      tile_start, n_unique, n_brt_host);
+      tile_start, n_unique, n_brt_host, ctrl);
 }
 
 }  // namespace ocp

@@ -41,6 +41,7 @@ __global__ __launch_bounds__(256) void unique_kernel(
   const uint32_t lane = lane_id();
   const int32_t t = blockIdx.x;  // in-order dispatch: see lookback_n
   const uint32_t tag = ctrl[kCtrlEpoch] * 8u + kStageUnique;
+  if (stopped(ctrl, tag)) return;
   const TileRef r = find_tile(tile_start, sv.n_seg, t);
   const int32_t sb = seg_begin(sv, r.seg);
   const int32_t se = seg_end(sv, r.seg);
@@ -87,9 +88,10 @@ __global__ __launch_bounds__(256) void unique_kernel(
     bool late = false;
     // Group sums live after the grid's tile flags.
     const uint32_t prev =
-        wave_lookback_groups(flags, flags + gridDim.x, t, t - r.local, total, tag, &late);
+        wave_lookback_groups(flags, flags + gridDim.x, t, t - r.local, total, tag,
+                             wait_limit(ctrl), &late);
     if (lane == 0) {
-      if (late) ctrl[kCtrlTimeout] = 1u;
+      if (late) raise_timeout(ctrl, tag);
       s_excl = prev;
       s_total = total;
       const int32_t n_tiles = (se - sb + T - 1) / T;

@@ -189,12 +189,12 @@ __device__ __forceinline__ uint64_t flag_load(const uint64_t* p) {
   return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
 }
-// Every wait below is bounded: after kSpinLimit unsuccessful polls the
-// caller gets `timed_out` set, publishes what it has and the kernel runs to
-// its end; the host turns the flag into an error. Tiles are numbered by
-// workgroup id, and workgroups are dispatched in id order on each XCD, so
-// the lowest unfinished tile is always resident and a wait ends long before
-// the limit.
+// Every wait below is bounded: after `limit` unsuccessful polls (kSpinLimit
+// unless the host sets another) the caller gets `timed_out` set, publishes
+// what it has and the kernel runs to its end; the host turns the flag into
+// an error. Tiles are numbered by workgroup id, and workgroups are
+// dispatched in id order on each XCD, so the lowest unfinished tile is
+// always resident and a wait ends long before the limit.
 constexpr uint32_t kSpinLimit = 1u << 22;
 
 // Lookback of one thread per chain (one digit of a OneSweep tile), polling
@@ -205,11 +205,11 @@ template <int W>
 __device__ __forceinline__ uint32_t lookback_n(const uint64_t* flags,
                                                uint32_t stride, int32_t t,
                                                int32_t first, uint32_t tag,
-                                               bool* timed_out) {
+                                               uint32_t limit, bool* timed_out) {
   uint32_t excl = 0;
   int32_t k = t - 1;
   const uint32_t want = tag & 0x3fffffffu;
-  for (uint32_t spin = 0; spin < kSpinLimit;) {
+  for (uint32_t spin = 0; spin < limit;) {
     uint64_t v[W];
 #pragma unroll
     for (int u = 0; u < W; ++u)
@@ -245,9 +245,9 @@ __device__ __forceinline__ uint32_t lookback_n(const uint64_t* flags,
 template <bool MAX = false>
 __device__ __forceinline__ uint32_t wave_sum_flags(const uint64_t* base, int32_t stride,
                                                    int32_t cnt, uint32_t want,
-                                                   bool* timed_out) {
+                                                   uint32_t limit, bool* timed_out) {
   const int32_t lane = static_cast<int32_t>(lane_id());
-  for (uint32_t spin = 0; spin < kSpinLimit; ++spin) {
+  for (uint32_t spin = 0; spin < limit; ++spin) {
     bool ready = true;
     uint32_t c = 0;
     if (lane < cnt) {
@@ -284,15 +284,16 @@ __device__ __forceinline__ uint32_t wave_sum_flags(const uint64_t* base, int32_t
 __device__ __forceinline__ uint32_t wave_lookback_groups(uint64_t* flags, uint64_t* gflags,
                                                          int32_t t, int32_t first,
                                                          uint32_t own, uint32_t tag,
-                                                         bool* timed_out) {
+                                                         uint32_t limit, bool* timed_out) {
   const uint32_t want = tag & 0x3fffffffu;
   const int32_t local = t - first;
   const int32_t g0 = local & ~63;  // this tile's group, as a local tile index
-  uint32_t excl = wave_sum_flags(flags + first + g0, 1, local - g0, want, timed_out);
+  uint32_t excl = wave_sum_flags(flags + first + g0, 1, local - g0, want, limit, timed_out);
   if ((local & 63) == 63 && lane_id() == 0)
     flag_store(gflags + first + g0, flag_pack(tag, kFlagAgg, excl + own));
   for (int32_t g = 0; g < g0; g += 64 * 64)
-    excl += wave_sum_flags(gflags + first + g, 64, min(64, (g0 - g) >> 6), want, timed_out);
+    excl += wave_sum_flags(gflags + first + g, 64, min(64, (g0 - g) >> 6), want, limit,
+                           timed_out);
   return excl;
 }
 

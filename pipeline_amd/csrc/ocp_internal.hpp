@@ -18,9 +18,38 @@ enum CtrlSlot : int {
   kCtrlOctOk = 10,     // 1 when the octree fits the node capacity
   kCtrlLearnFail = 11, // learned sort: buckets that overflowed
   kCtrlLearnBad = 12,  // learned sort: misordered bucket boundaries
-  kCtrlTimeout = 13,   // a lookback wait hit its spin limit (result invalid)
+  kCtrlTimeout = 13,   // mark of the launch whose wait gave up (0: none)
+  kCtrlWaitLimit = 14, // 0: waits poll up to kSpinLimit times; v > 0: v - 1
   kCtrlWords = 64,
 };
+
+// Polls a bounded wait may spend before it gives up (set from the host with
+// ocp_set_wait_limit; 0 makes every wait give up at once).
+__device__ __forceinline__ uint32_t wait_limit(const uint32_t* ctrl) {
+  const uint32_t v = ctrl[kCtrlWaitLimit];
+  return v ? v - 1u : kSpinLimit;
+}
+
+// A wait that gives up stores its launch's mark (below) in kCtrlTimeout;
+// the first one wins, later ones store the same mark. The keys such a pass
+// leaves may no longer match the digit counts the next pass is sized by, so
+// every later kernel that consumes lookback output does nothing until the
+// host has reported the error and cleared the word. A kernel stops only on
+// the mark of an earlier launch: that value cannot change while it runs
+// (tiles of the running launch can only store its own mark), so every
+// thread of every workgroup takes the same branch.
+__device__ __forceinline__ uint32_t timeout_mark(uint32_t tag) { return tag | 0x80000000u; }
+__device__ __forceinline__ bool stopped(const uint32_t* ctrl, uint32_t tag) {
+  const uint32_t v = ctrl[kCtrlTimeout];
+  return v != 0u && v != timeout_mark(tag);
+}
+__device__ __forceinline__ void raise_timeout(uint32_t* ctrl, uint32_t tag) {
+  ctrl[kCtrlTimeout] = timeout_mark(tag);
+}
+// Kernels without waits stop on any mark.
+__device__ __forceinline__ bool stopped_any(const uint32_t* ctrl) {
+  return ctrl[kCtrlTimeout] != 0u;
+}
 
 // Lookback stage ids (ticket counter and flag tag).
 enum Stage : uint32_t {
@@ -96,13 +125,14 @@ void launch_unique(const uint32_t* in, uint32_t* out, const SegView& sv,
 // n_brt: per-segment node counts; when n_unique != nullptr the count of
 // segment k is n_unique[k]-1, else n_brt_host (single segment).
 // parent_pn (optional): prefix_n[parent[i]] at i, written with parent[].
+// ctrl (optional): the kernel does nothing once a lookback wait gave up.
 void launch_radix_tree(const uint32_t* codes, uint8_t* prefix_n,
                        uint8_t* has_leaf_left, uint8_t* has_leaf_right,
                        int32_t* left_child, int32_t* parent, uint8_t* parent_pn,
                        const SegView& sv,
                        const int32_t* tile_start, int32_t total_tiles,
                        const int32_t* n_unique, int32_t n_brt_host,
-                       hipStream_t s);
+                       const uint32_t* ctrl, hipStream_t s);
 
 // ---- stages 5 + 6 (k_edge_scan.hip)
 // mode 0: edge count + inclusive scan fused; 1: edge count only;
@@ -142,7 +172,8 @@ void launch_oct_root(OctNode* nodes, const uint32_t* codes,
 // gaps (the fused pipeline passes 1|2|4|8).
 // Needs every OctArgs array (the fused build has them all).
 constexpr int kOctMake = 1, kOctLink = 2, kOctRoot = 4, kOctFull = 8;
-void launch_oct_build(const OctArgs& a, const SegView& sv,
+// Returns false (and launches nothing) when an array it needs is null.
+bool launch_oct_build(const OctArgs& a, const SegView& sv,
                       const int32_t* tile_start, int32_t total_tiles, int mode,
                       hipStream_t s);
 

@@ -76,6 +76,8 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
 
   const int32_t t = blockIdx.x;
   if (tiles_dev && t >= *tiles_dev) return;
+  const uint32_t tag = ctrl[kCtrlEpoch] * 8u + stage;
+  if (stopped(ctrl, tag)) return;
   const int tid = threadIdx.x;
   const uint32_t w = wave_id();
   const uint32_t lane = lane_id();
@@ -83,7 +85,6 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
   for (int i = tid; i < WAVES * 256; i += THREADS) s_whist[i] = 0;
   dig.setup(r.seg, tid, THREADS);
   __syncthreads();
-  const uint32_t tag = ctrl[kCtrlEpoch] * 8u + stage;
   const int32_t sb = seg_begin(sv, r.seg);
   const int32_t base = sb + r.local * T;
   const int32_t cnt = min(T, seg_end(sv, r.seg) - base);
@@ -170,8 +171,9 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
     uint32_t prev = 0;
     if (r.local > 0) {
       bool late = false;
-      prev = lookback_n<kOneSweepWindow>(flags + tid, 256, t, t - r.local, tag, &late);
-      if (late) ctrl[kCtrlTimeout] = 1u;
+      prev = lookback_n<kOneSweepWindow>(flags + tid, 256, t, t - r.local, tag,
+                                         wait_limit(ctrl), &late);
+      if (late) raise_timeout(ctrl, tag);
       flag_store(&flags[static_cast<int64_t>(t) * 256 + tid],
                  flag_pack(tag, kFlagInc, prev + tile_cnt));
     }

@@ -206,9 +206,12 @@ def _ref_tree(codes):
     return t
 
 
-@pytest.mark.skipif(oracle.ref_lib() is None, reason="oracle/_ref not built (no reference tree)")
 def test_stages_match_compiled_reference():
+    # Looked up inside the test, not at collection: a GPU-only run never
+    # maps the compiled reference.
     R = oracle.ref_lib()
+    if R is None:
+        pytest.skip("oracle/_ref not built (no reference tree)")
     rng = np.random.default_rng(11)
     for kind in range(3):
         n = 50000
