@@ -17,6 +17,9 @@ quantisation never wraps (include/shared/morton.h:42-44 maps 1024 to 0).
 """
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 LIMIT = np.nextafter(np.float32(1024.0), np.float32(0.0))
@@ -76,28 +79,43 @@ def cube_box(pts: np.ndarray):
 
 
 def batch(n_clouds: int = 512, lo: int = 4096, hi: int = 2 * 1024 * 1024,
-          seed: int = 114514, float4: bool = True, max_total: int | None = None):
-    """Returns (points, seg_offsets[n+1], seg_min[n], seg_range[n], kinds)."""
+          seed: int = 114514, float4: bool = True, max_total: int | None = None,
+          workers: int | None = None):
+    """Returns (points, seg_offsets[n+1], seg_min[n], seg_range[n], kinds).
+    Clouds are generated on `workers` threads (each from its own seed; the
+    batch's own draws come first), so the result does not depend on it."""
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), n_clouds)).astype(np.int64)
     if max_total is not None and sizes.sum() > max_total:
         sizes = np.maximum((sizes * (max_total / sizes.sum())).astype(np.int64), 1)
     kinds = rng.integers(0, 2, n_clouds)
+    scales = np.zeros(n_clouds)
+    shifts = np.zeros((n_clouds, 3))
+    for k in range(n_clouds):
+        scales[k] = rng.uniform(0.05, 4.0)
+        shifts[k] = rng.uniform(-2000.0, 2000.0, 3)
     offs = np.zeros(n_clouds + 1, np.int64)
     offs[1:] = np.cumsum(sizes)
     width = 4 if float4 else 3
     pts = np.zeros((int(offs[-1]), width), np.float32)
     smin = np.zeros(n_clouds, np.float32)
     srg = np.zeros(n_clouds, np.float32)
-    for k in range(n_clouds):
-        n = int(sizes[k])
+
+    def one(k: int) -> None:
         gen = clustered if kinds[k] else uniform
-        c = gen(n, seed=seed * 7919 + k, float4=False).astype(np.float64)
-        scale = rng.uniform(0.05, 4.0)
-        shift = rng.uniform(-2000.0, 2000.0, 3)
-        c = (c * scale + shift).astype(np.float32)
+        c = gen(int(sizes[k]), seed=seed * 7919 + k, float4=False).astype(np.float64)
+        c = (c * scales[k] + shifts[k]).astype(np.float32)
         pts[offs[k]:offs[k + 1], :3] = c
         smin[k], srg[k] = cube_box(c)
+
+    if workers is None:
+        workers = min(16, os.cpu_count() or 1)
+    if workers <= 1:
+        for k in range(n_clouds):
+            one(k)
+    else:
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            list(ex.map(one, range(n_clouds)))
     return pts, offs.astype(np.int32), smin, srg, kinds
 
 

@@ -1,9 +1,12 @@
 """GPU parity of the fused device-resident build (plain, graph replay,
 batched) against the oracle's full chain, stage output by stage output."""
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 import pytest
 
 import oracle
+from octree_checks import check_octree
 from pipeline_amd import clouds
 
 pytestmark = pytest.mark.gpu
@@ -103,12 +106,62 @@ def test_batched_matches_per_cloud_oracle(ctx, graph):
         d.free()
 
 
+def _segment_bytes(out, s):
+    return [out.sorted_keys[s], out.unique_keys[s]] + [getattr(out, k)[s] for k in STAGES] + \
+        [out.nodes[s].view(np.uint8), out.n_unique[s:s + 1], out.n_oct[s:s + 1]]
+
+
+@pytest.mark.timeout(900)
+def test_batched_512_clouds_full_workload(ctx):
+    """The batched workload at its real size: 512 clouds of 4,096 to ~2M
+    points (log-uniform, uniform or clustered, each with its own box), one
+    segmented launch per stage. Every cloud above 262,144 points (64 scan
+    tiles, where the two-level lookback of a later segment starts) and 64
+    others are compared byte for byte with the oracle; a graph replay must
+    give the eager build's bytes for every cloud."""
+    pts, offs, smin, srg, _ = clouds.batch(512, seed=114514)
+    sizes = np.diff(offs)
+    d = ctx.upload(pts)
+    try:
+        ctx.set_graph_mode(False)
+        ctx.build_batched(d, offs, smin, srg)
+        eager = ctx.fetch(offs)
+        big = np.nonzero(sizes > 262144)[0]
+        rest = np.nonzero(sizes <= 262144)[0]
+        picked = np.concatenate([big, rest[np.linspace(0, rest.size - 1, 64).astype(int)]])
+        assert big.size > 100 and picked.size == big.size + 64
+
+        def one(s):
+            seg = pts[offs[s]:offs[s + 1]]
+            _check_segment(eager, s, oracle.build_plain(seg, float(smin[s]), float(srg[s])))
+            return s
+
+        with ThreadPoolExecutor(max_workers=8) as ex:
+            done = list(ex.map(one, picked.tolist()))
+        assert len(done) == picked.size
+        ctx.set_graph_mode(True)
+        for _ in range(2):  # capture, then replay
+            ctx.build_batched(d, offs, smin, srg)
+            rep = ctx.fetch(offs)
+            for s in range(512):
+                for a, b in zip(_segment_bytes(rep, s), _segment_bytes(eager, s)):
+                    assert np.array_equal(a, b), f"cloud {s}"
+    finally:
+        ctx.set_graph_mode(False)
+        d.free()
+
+
 def test_plain_10M_uniform_full_oracle(ctx):
     """The plain GPU workload at its real size: 10M uniform float4 points,
-    seed 114514, every stage output against the oracle."""
+    seed 114514, every stage output against the oracle, plus the octree's
+    order-independent structure (written slots, zero gaps, grid-aligned
+    corners, power-of-two cells) checked without the oracle."""
     pts = clouds.uniform(10_000_000, seed=114514)
     exp = oracle.build_plain(pts)
-    _check_segment(_plain(ctx, pts), 0, exp)
+    out = _plain(ctx, pts)
+    check_octree(out.nodes[0], out.unique_keys[0], out.prefix_n[0], out.edge_count[0],
+                 out.offsets[0])
+    _check_segment(out, 0, exp)
 
 
 def test_plain_2p24_clustered_full_oracle(ctx):
@@ -117,7 +170,10 @@ def test_plain_2p24_clustered_full_oracle(ctx):
     pts = clouds.clustered(1 << 24, seed=114514)
     exp = oracle.build_plain(pts)
     assert exp["n_unique"] < (1 << 24) // 2  # the duplicate-heavy case
-    _check_segment(_plain(ctx, pts), 0, exp)
+    out = _plain(ctx, pts)
+    check_octree(out.nodes[0], out.unique_keys[0], out.prefix_n[0], out.edge_count[0],
+                 out.offsets[0])
+    _check_segment(out, 0, exp)
 
 
 def test_tuned_configs_give_identical_outputs(ctx):

@@ -229,3 +229,51 @@ def test_gpu_cdf_save_load_roundtrip(ctx, tmp_path):
     bad.write_bytes(raw[:100])
     with pytest.raises(Exception):
         ctx.cdf_load(str(bad))
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(900)
+def test_gpu_sort_learned_2p26_moving_frames(ctx):
+    """The learned-sort workload at its real size: 2^26 clustered points,
+    three successive frames of small rigid motions. Frame t's keys are
+    sorted with the model trained on 1% samples of the frames before it and
+    compared with numpy's sort and with OneSweep's bytes; the trained model
+    leaves fewer keys to the fallback than the linear CDF."""
+    n = 1 << 26
+    cur = clouds.clustered(n, seed=114514)
+    dk = ctx.empty(n, np.uint32)
+    alt = ctx.empty(n, np.uint32)
+    ref = ctx.empty(n, np.uint32)
+    ctx.cdf_clear_samples()
+    prev = None
+    try:
+        for t in range(3):
+            if t:
+                cur = clouds.rigid_motion(cur, seed=1000 + t)
+            keys = morton_keys(cur)
+            if prev is not None:
+                ctx.cdf_add_samples(prev, n, 100)
+                _ = ctx.cdf_train(steps=2000, lr=1.0, momentum=0.9, seed=t)
+                dk.copy_from(keys)
+                st = ctx.sort_learned(dk, alt, n)
+                got = dk.to_host()
+                np.testing.assert_array_equal(got, np.sort(keys), err_msg=f"frame {t}")
+                ref.copy_from(keys)
+                ctx.k_SortKeys(ref, alt, n)
+                ctx.synchronize()
+                assert np.array_equal(ref.to_host(), got), f"frame {t} vs OneSweep"
+                assert st["misordered"] == 0 and st["full_resort"] == 0, st
+                if t == 2:
+                    ctx.cdf_reset(int(keys.min()), int(keys.max()))
+                    dk.copy_from(keys)
+                    st_lin = ctx.sort_learned(dk, alt, n)
+                    assert np.array_equal(dk.to_host(), got)
+                    print("trained", st, "linear", st_lin, flush=True)
+                    assert st["keys_big"] < st_lin["keys_big"]
+            if prev is not None:
+                prev.free()
+            prev = ctx.upload(keys)
+    finally:
+        for a in (dk, alt, ref, prev):
+            if a is not None:
+                a.free()

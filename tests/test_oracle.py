@@ -17,6 +17,8 @@ import numpy as np
 import pytest
 
 import oracle
+from octree_checks import check_octree
+from pipeline_amd import clouds
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = os.path.join(HERE, "golden", "golden.npz")
@@ -304,6 +306,108 @@ def test_octree_hand_traced_short_root():
     assert k == 1
     exp = _node([1, 0, 0, 0, 0, 0, 0, 0], [0, 0, 0, 1], 1024.0, 0)
     np.testing.assert_array_equal(nodes.view(np.uint8), exp.view(np.uint8))
+
+
+def test_octree_hand_traced_overlapping_chains():
+    """codes 0, 1, 512, 4096, 2^21: a chain of three slots overlapped by the
+    next, shorter chain, traced by hand through 04_radix_tree.cpp:39-121,
+    05_edge_count.cpp:3-16, 06_partial_sum.cpp:3-15, the root set-up of
+    apps/cpu_demo/main.cpp:125-132 and oct.h:24-134. Pins the per-step level
+    of the make loop (j = 0, 1), the last writer of a corner and cell size
+    where chains overlap, a make-phase child that the link pass overwrites,
+    the 31-step walk to the root and the leaf shift of level 11."""
+    codes = np.array([0, 1, 512, 4096, 1 << 21], np.uint32)
+    t = oracle.build_radix_tree(codes)
+    # delta: (c0,c1)=30, (c0|c1,c2)=21, (c0..c2,c3)=18, (*,c4)=9.
+    # node 0: root quirk, range [0, 3], delta 18, split after key 2 (delta(c0,c2)
+    #   = 21 > 18, delta(c0,c3) = 18 is not): gamma 2, right leaf 3, parent[2]=0
+    # node 1: d = -1, range [0, 1], delta 30, both children leaves 0 and 1
+    # node 2: d = -1, range [0, 2], delta 21, gamma 1 (delta(c2,c1) = 21 is not
+    #   > 21): left child node 1 (parent[1] = 2), right leaf 2
+    # node 3: d = -1, range [0, 3], delta 18, gamma 2: left child node 2
+    #   (parent[2] = 3 overwrites node 0's write), right leaf 3
+    np.testing.assert_array_equal(t["prefix_n"], [18, 30, 21, 18])
+    np.testing.assert_array_equal(t["has_leaf_left"], [0, 1, 0, 0])
+    np.testing.assert_array_equal(t["has_leaf_right"], [1, 1, 1, 1])
+    np.testing.assert_array_equal(t["left_child"], [2, 0, 1, 2])
+    np.testing.assert_array_equal(t["parent"], [0, 2, 3, 0])
+    e = oracle.edge_count(t["prefix_n"], t["parent"])
+    np.testing.assert_array_equal(e, [0, 3, 1, 0])   # 10-7, 7-6, 6-6
+    p = oracle.partial_sum(e)
+    p[0] = 0
+    np.testing.assert_array_equal(p, [0, 3, 4, 4])   # chain 1: slots 3..5, chain 2: slot 4
+    nodes, k = oracle.octree(codes, t, e, p)
+    assert k == 6                                    # max(P + e) = 3 + 3
+    # root (level 18/3 = 6): corner decode(0), cell 1024.
+    # i = 1, j = 0: level 10, child (1 >> 0) & 7 = 1 -> slot 4 children[1] = 3;
+    #   slot 3 corner decode(1) = (1,0,0), cell 1024 / 2^(10-6) = 64.
+    # i = 1, j = 1: level 9, child (1 >> 3) & 7 = 0 -> slot 5 children[0] = 4;
+    #   slot 4 corner (0,0,0), cell 128.
+    # i = 1, top: parent[1] = 2 has e = 1, so slot P[2] = 4, top level
+    #   10 - 3 + 1 = 8, child (1 >> 6) & 7 = 0 -> slot 4 children[0] = 5;
+    #   slot 5 corner (0,0,0), cell 256. Slot 4 mask = 0b11.
+    # i = 2, top: parent 3 (e = 0) -> 0 (e = 0) -> 0 ... capped at 31 steps:
+    #   slot P[0] = 0, level 7, child (512 >> 9) & 7 = 1 -> slot 0 children[1]
+    #   = 4; slot 4 corner decode(512) = (8,0,0), cell 512: chain 2 is the last
+    #   writer of slot 4's geometry.
+    # links: node 0 right leaf 3 at level 7: child (4096 >> 9) & 7 = 0, walk
+    #   stops at the root -> slot 0 children[0] = 3. Node 1 leaves 0 and 1 at
+    #   level 11: shift (30 - 33) & 31 = 29, child 0 -> slot 3 children[0] = 0,
+    #   then = 1. Node 2 right leaf 2 at level 8: child (512 >> 6) & 7 = 0 ->
+    #   slot 4 children[0] = 2, overwriting the make-phase 5. Node 3 right leaf
+    #   3 -> slot 0 children[0] = 3 again. Slots 1 and 2: never written.
+    exp = np.concatenate([
+        _node([3, 4, 0, 0, 0, 0, 0, 0], [0, 0, 0, 1], 1024.0, 0b10),
+        _node([0] * 8, [0, 0, 0, 0], 0.0, 0),
+        _node([0] * 8, [0, 0, 0, 0], 0.0, 0),
+        _node([1, 0, 0, 0, 0, 0, 0, 0], [1, 0, 0, 1], 64.0, 0),
+        _node([2, 3, 0, 0, 0, 0, 0, 0], [8, 0, 0, 1], 512.0, 0b11),
+        _node([4, 0, 0, 0, 0, 0, 0, 0], [0, 0, 0, 1], 256.0, 0b1),
+    ])
+    np.testing.assert_array_equal(nodes.view(np.uint8), exp.view(np.uint8))
+    assert check_octree(nodes, codes, t["prefix_n"], e, p) == 4
+
+
+@pytest.mark.parametrize("kind,n", [("uniform", 200000), ("clustered", 200000),
+                                    ("uniform", 5), ("clustered", 4096)])
+def test_oracle_octree_structure(kind, n):
+    """The oracle's octree output satisfies the order-independent structure
+    of the reference's semantics (tests/octree_checks.py)."""
+    gen = clouds.uniform if kind == "uniform" else clouds.clustered
+    r = oracle.build_plain(gen(n, seed=n + 3))
+    written = check_octree(r["nodes"], r["unique"], r["prefix_n"], r["edge_count"],
+                           r["offsets"])
+    assert written >= 1
+
+
+def test_octree_structure_check_catches_planted_mistakes():
+    r = oracle.build_plain(clouds.clustered(20000, seed=9))
+    args = (r["unique"], r["prefix_n"], r["edge_count"], r["offsets"])
+    check_octree(r["nodes"], *args)
+    written = np.nonzero(r["nodes"]["cell_size"] != 0)[0]
+    s = int(written[len(written) // 2])
+    bad = r["nodes"].copy()
+    bad["corner"][s, 0] += np.float32(0.5)           # off the grid
+    with pytest.raises(AssertionError):
+        check_octree(bad, *args)
+    bad = r["nodes"].copy()
+    bad["cell_size"][s] *= np.float32(0.75)          # not range / 2^k
+    with pytest.raises(AssertionError):
+        check_octree(bad, *args)
+    bad = r["nodes"].copy()
+    bad[s] = np.zeros(1, bad.dtype)[0]               # a covered slot left zero
+    with pytest.raises(AssertionError):
+        check_octree(bad, *args)
+    gaps = np.nonzero(r["nodes"]["cell_size"] == 0)[0]
+    if gaps.size:
+        bad = r["nodes"].copy()
+        bad["children"][int(gaps[0]), 3] = 7         # a gap written
+        with pytest.raises(AssertionError):
+            check_octree(bad, *args)
+    bad = r["nodes"].copy()
+    bad["child_leaf_mask"][s] = 1
+    with pytest.raises(AssertionError):
+        check_octree(bad, *args)
 
 
 def test_single_and_empty_clouds():
