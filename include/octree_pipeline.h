@@ -117,6 +117,18 @@ int ocp_k_DigitBinning(ocp_ctx ctx, const uint32_t* global_histogram,
 int ocp_k_SortKeys(ocp_ctx ctx, uint32_t* keys, uint32_t* keys_alt,
                    uint32_t n);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ocp_k_SortKeys with every scratch buffer (digit counts,
+ * lookback flags and their epoch) in a fixed array of the library's device
+ * memory, 8 MiB per device, set up once per device by ocp_create. Makes no
+ * allocation, frees nothing. At most 2^24 keys (OCP_ERR_CAPACITY beyond).
+ * All calls on one device share that array and keep its last contents:
+ * calls that overlap on one device, from any context or stream, race. */
+#define OCP_STATIC_SORT_MAX_KEYS (1u << 24)
+int ocp_k_SortKeys_static(ocp_ctx ctx, uint32_t* keys, uint32_t* keys_alt,
+                          uint32_t n);
+
 /* ------------------------------------------------------------ stage 3 */
 /* Replaces gpu::k_CountUnique (include/cuda/kernels/03_unique.cuh:5) and
  * k_CountUnique (include/kernels/03_unique.hpp:4): in-place unique of n

@@ -76,6 +76,7 @@ class LearnedStats(ctypes.Structure):
         return {f: int(getattr(self, f)) for f, _ in self._fields_ if f != "reserved"}
 
 
+STATIC_SORT_MAX_KEYS = 1 << 24
 CDF_PARAMS = 12338
 CDF_ROOT_PARAMS = 49
 CDF_BUCKETS = 16384
@@ -101,6 +102,7 @@ SIGNATURES = {
     "ocp_k_GlobalHistogram": (_INT, [_P, _P, _P, _U32]),
     "ocp_k_DigitBinning": (_INT, [_P, _P, _P, _P, _U32, _U32]),
     "ocp_k_SortKeys": (_INT, [_P, _P, _P, _U32]),
+    "ocp_k_SortKeys_static": (_INT, [_P, _P, _P, _U32]),
     "ocp_k_CountUnique": (_INT, [_P, _P, _P, _I32]),
     "ocp_k_BuildRadixTree": (_INT, [_P, _I32, _P, _P, _P, _P, _P, _P]),
     "ocp_k_EdgeCount": (_INT, [_P, _P, _P, _P, _I32]),
@@ -319,6 +321,11 @@ class Context:
         self._check(self.lib.ocp_k_SortKeys(self.h, keys.ptr, keys_alt.ptr, n))
 
     k_SimpleRadixSort = k_SortKeys
+
+    def k_SortKeys_static(self, keys: DeviceArray, keys_alt: DeviceArray, n: int):
+        """ocp_k_SortKeys with its scratch in the library's fixed device array
+        (no allocation; at most STATIC_SORT_MAX_KEYS keys)."""
+        self._check(self.lib.ocp_k_SortKeys_static(self.h, keys.ptr, keys_alt.ptr, n))
 
     def k_CountUnique(self, keys: DeviceArray, n: int) -> int:
         out = self.empty(1, np.int32)

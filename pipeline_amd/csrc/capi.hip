@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -105,6 +106,51 @@ struct LearnState {
   hipEvent_t ev_info = nullptr;
 };
 constexpr int64_t kSampleCap = int64_t(1) << 22;
+
+// ---- Scratch of ocp_k_SortKeys_static, the sort that allocates nothing.
+// One fixed array per device, in the code object's device memory: a 256-byte
+// header (word 0: the device-wide epoch of the flags below), the 4 x 256
+// digit counts, then the lookback flags of up to kStaticFlagWords / 256
+// tiles - enough for kStaticSortMaxKeys keys at the smallest sort tile
+// (4096 keys). 8 MiB + 4.25 KiB per device, held until the process ends.
+constexpr int64_t kStaticSortMaxKeys = int64_t(1) << 24;
+constexpr int64_t kStaticHeaderBytes = 256;
+constexpr int64_t kStaticHistBytes = 1024 * sizeof(uint32_t);
+constexpr int64_t kStaticFlagWords = (kStaticSortMaxKeys / 4096) * 256;
+constexpr int64_t kStaticScratchBytes =
+    kStaticHeaderBytes + kStaticHistBytes + kStaticFlagWords * int64_t(sizeof(uint64_t));
+static_assert((kStaticHeaderBytes + kStaticHistBytes) % 256 == 0, "flags stay 256-byte aligned");
+
+}  // namespace
+
+// In host code this name is only a stand-in for the symbol; the memory is
+// reached through the per-device address that hipGetSymbolAddress returns.
+static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_ptr[kMaxDevices] = {};
+
+// Once per device, with that device current: the device address of
+// `scratch`, whose flags are then cleared (no stale flag may carry a tag).
+// Called from ocp_create, the library's set-up call.
+int scratch_init(int device, hipStream_t stream, std::string* why) {
+  if (device < 0 || device >= kMaxDevices) return *why = "device index out of range", OCP_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return OCP_OK;
+  void* ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return *why = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e), OCP_ERR_HIP;
+  launch_fill_u32(static_cast<uint32_t*>(ptr), 0u, kStaticScratchBytes / 4, stream);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) return *why = std::string("scratch clear: ") + hipGetErrorString(e), OCP_ERR_HIP;
+  g_scratch_ptr[device] = static_cast<uint8_t*>(ptr);
+  return OCP_OK;
+}
 
 }  // namespace
 
@@ -625,6 +671,11 @@ int ocp_create(ocp_ctx* out, int device, void* stream) {
     ocp_destroy(c);
     return OCP_ERR_HIP;
   }
+  std::string why;
+  if (int r = scratch_init(device, c->stream, &why)) {
+    ocp_destroy(c);
+    return r;
+  }
   for (auto& e : c->ev) (void)hipEventCreate(&e);
   for (auto& e : c->kev) (void)hipEventCreate(&e);
   *out = c;
@@ -779,6 +830,35 @@ int ocp_k_SortKeys(ocp_ctx c, uint32_t* keys, uint32_t* alt, uint32_t n) {
   for (int pass = 0; pass < 4; ++pass) {
     launch_onesweep_pass(src, dst, sv, nullptr, tiles, c->s_hist, c->s_flags, c->ctrl, pass,
                          pass != 0, c->tune, c->stream);
+    std::swap(src, dst);
+  }
+  OCP_CHECK_LAUNCH(c);
+  return OCP_OK;
+}
+
+int ocp_k_SortKeys_static(ocp_ctx c, uint32_t* keys, uint32_t* alt, uint32_t n) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (n <= 1) return OCP_OK;
+  if (!keys || !alt) return fail(c, OCP_ERR_INVALID, "null pointer");
+  if (n > kStaticSortMaxKeys)
+    return fail(c, OCP_ERR_CAPACITY, "ocp_k_SortKeys_static sorts at most 2^24 keys");
+  const int32_t tiles = static_cast<int32_t>(cdiv(n, sort_tile(c->tune)));
+  if (int64_t(tiles) * 256 > kStaticFlagWords)
+    return fail(c, OCP_ERR_CAPACITY, "ocp_k_SortKeys_static: too many sort tiles");
+  uint8_t* base = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch_ptr[c->device] : nullptr;
+  if (!base) return fail(c, OCP_ERR_STATE, "static sort scratch not set up for this device");
+  uint32_t* dev_epoch = reinterpret_cast<uint32_t*>(base);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(base + kStaticHeaderBytes);
+  uint64_t* flags = reinterpret_cast<uint64_t*>(base + kStaticHeaderBytes + kStaticHistBytes);
+  const SegView sv = single_view(n, 0.0f, 1.0f);
+  launch_begin_shared(c->ctrl, dev_epoch, hist, 1024, c->stream);
+  launch_histogram(keys, sv, nullptr, static_cast<int32_t>(cdiv(n, morton_tile(c->tune))), hist,
+                   c->tune, c->stream);
+  uint32_t* src = keys;
+  uint32_t* dst = alt;
+  for (int pass = 0; pass < 4; ++pass) {
+    launch_onesweep_pass(src, dst, sv, nullptr, tiles, hist, flags, c->ctrl, pass, pass != 0,
+                         c->tune, c->stream);
     std::swap(src, dst);
   }
   OCP_CHECK_LAUNCH(c);

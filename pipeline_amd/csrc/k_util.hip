@@ -32,6 +32,24 @@ __global__ __launch_bounds__(256) void begin_kernel(uint32_t* ctrl,
   for (int64_t i = tid; i < n2; i += stride) z2[i] = 0;
 }
 
+// Begin of a sort whose lookback flags live in memory shared by every
+// context on the device: the context's epoch moves past the device-wide one
+// kept beside those flags, and both take the new value, so a flag left by
+// another context's earlier sort never carries a tag this sort waits for.
+__global__ __launch_bounds__(256) void begin_shared_kernel(uint32_t* ctrl, uint32_t* dev_epoch,
+                                                           uint32_t* hist, int64_t n_hist) {
+  const int64_t tid = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) ctrl[kCtrlTicket0 + k] = 0;
+    const uint32_t e = max(ctrl[kCtrlEpoch], *dev_epoch) + 1u;
+    ctrl[kCtrlEpoch] = e;
+    *dev_epoch = e;
+  }
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = tid; i < n_hist; i += stride) hist[i] = 0;
+}
+
 // Exclusive scan of the per-segment slot counts into node offsets; one
 // workgroup of 1024 threads walks the segments in chunks.
 __global__ __launch_bounds__(1024) void oct_offsets_kernel(
@@ -103,6 +121,12 @@ void launch_begin(uint32_t* ctrl, uint32_t* hist, int64_t n_hist,
   const int64_t blocks = std::max<int64_t>(
       1, std::min<int64_t>((std::max(n_hist, n2) + 255) / 256, 2048));
   begin_kernel<<<static_cast<int>(blocks), 256, 0, s>>>(ctrl, hist, n_hist, z2, n2);
+}
+
+void launch_begin_shared(uint32_t* ctrl, uint32_t* dev_epoch, uint32_t* hist, int64_t n_hist,
+                         hipStream_t s) {
+  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n_hist + 255) / 256, 64));
+  begin_shared_kernel<<<static_cast<int>(blocks), 256, 0, s>>>(ctrl, dev_epoch, hist, n_hist);
 }
 
 void launch_oct_offsets(const int32_t* kalloc, int32_t n_seg,

@@ -86,6 +86,10 @@ void launch_fill_u32(uint32_t* p, uint32_t value, int64_t n_words,
 // n2 words at z2.
 void launch_begin(uint32_t* ctrl, uint32_t* hist, int64_t n_hist,
                   hipStream_t s, uint32_t* z2 = nullptr, int64_t n2 = 0);
+// Begin for lookback flags shared by every context on the device: the new
+// epoch is past both the context's and *dev_epoch, and is stored in both.
+void launch_begin_shared(uint32_t* ctrl, uint32_t* dev_epoch, uint32_t* hist, int64_t n_hist,
+                         hipStream_t s);
 void launch_oct_offsets(const int32_t* kalloc, int32_t n_seg,
                         int32_t* oct_start, uint32_t* ctrl, int64_t k_cap,
                         hipStream_t s);
