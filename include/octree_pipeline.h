@@ -83,6 +83,15 @@ int ocp_memset(ocp_ctx ctx, void* dst, int value, size_t bytes);
  * pointers, size a multiple of 16). */
 int ocp_copy(ocp_ctx ctx, void* dst, const void* src, size_t bytes);
 
+/* ------------------------------------------------------------ input */
+/* Replaces gpu::k_InitRandomVec4 (include/cuda/kernels/00_init.cuh:7-8,
+ * kernels/cuda/00_init.cu:27-50): n float4 points from the hybrid
+ * LCG-Tausworthe generator, one step per point from state seeded by its
+ * index - the reference's output when launched with at least n threads
+ * (apps/gpu_demo/main.cu:167-173). Coordinates (and w) in [min, min+range). */
+int ocp_k_InitRandomVec4(ocp_ctx ctx, float* data, int32_t n, float min_coord,
+                         float range, int32_t seed);
+
 /* ------------------------------------------------------------ stage 1 */
 /* Replaces gpu::k_ComputeMortonCode (include/cuda/kernels/01_morton.cuh:7-11)
  * and k_ComputeMortonCode (include/kernels/01_morton.hpp:9-13).
@@ -197,15 +206,24 @@ int ocp_build(ocp_ctx ctx, const float* points, int float3, int64_t n,
               float min_coord, float range);
 /* Batched form: n_seg independent clouds, cloud k = points
  * [seg_offsets[k], seg_offsets[k+1]) with its own (seg_min[k],
- * seg_range[k]); one segmented launch per stage. Host arrays. */
+ * seg_range[k]); one segmented launch per stage. Host arrays. With seg_min
+ * and seg_range both NULL, each cloud's cube box is reduced on the device
+ * (min over x, y, z; range = max((max - min) * 1.0001, 1e-3), so the far
+ * face quantises below 1024) and read back through ocp_results. */
 int ocp_build_batched(ocp_ctx ctx, const float* points, int float3,
                       const int32_t* seg_offsets, const float* seg_min,
                       const float* seg_range, int32_t n_seg);
 /* The checks ocp_build_batched applies to its table, without a context or
  * device: offsets start at 0 and do not decrease, each segment holds at most
- * 2^27 points, the batch at most 2^30-1, ranges are > 0. */
+ * 2^27 points, the batch at most 2^30-1, ranges are > 0 (boxes may be both
+ * NULL). */
 int ocp_check_batch(const int32_t* seg_offsets, const float* seg_min,
                     const float* seg_range, int32_t n_seg);
+/* ocp_build with the cloud's cube box reduced on the device (as for
+ * ocp_build_batched with NULL boxes): no host value is needed before the
+ * build. The reference fixes the box at [0, 1024) (benchmarks/config.hpp:
+ * 7-10). */
+int ocp_build_autobox(ocp_ctx ctx, const float* points, int float3, int64_t n);
 /* Capture each distinct build shape once into a hipGraph and replay it. */
 int ocp_set_graph_mode(ocp_ctx ctx, int enable);
 
@@ -227,6 +245,8 @@ typedef struct ocp_results {
   const int32_t* oct_start;       /* device, n_seg + 1 */
   int32_t n_seg;
   int64_t n_points;
+  const float* seg_min;           /* device, n_seg: the boxes a batched or */
+  const float* seg_range;         /* autobox build used (NULL otherwise)   */
 } ocp_results;
 int ocp_get_results(ocp_ctx ctx, ocp_results* out);
 /* Synchronises and copies the per-segment U and K (host arrays of n_seg). */

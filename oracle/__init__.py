@@ -116,6 +116,37 @@ def ref_lib():
     return _ref
 
 
+# ---------------------------------------------------------------- generator
+def gpu_random_vec4(n: int, min_coord: float, range_: float, seed: int) -> np.ndarray:
+    """kernels/cuda/00_init.cu:27-50 launched with at least n threads
+    (apps/gpu_demo/main.cu:167-173): point i takes one hybrid LCG-Tausworthe
+    step from z_k = (4i + k) * seed (mod 2^32); coordinates are
+    min + ((x & 0xffffff) / 2^24) * range in float32, each product and sum
+    rounded separately. Not a parity item of the pipeline (it only feeds it);
+    parity unpinned beyond this restatement of the published algorithm."""
+    i = np.arange(n, dtype=np.uint64)
+    sd = np.uint64(np.uint32(np.int64(seed) & 0xffffffff))
+    m32 = np.uint64(0xffffffff)
+    z = [((i * np.uint64(4) + np.uint64(k)) * sd) & m32 for k in range(4)]
+
+    def taus(v, s1, s2, s3, msk):
+        return ((((v & np.uint64(msk)) << np.uint64(s3)) & m32) ^
+                ((((v << np.uint64(s1)) & m32) ^ v) >> np.uint64(s2)))
+    z1 = taus(z[0], 13, 19, 12, 4294967294)
+    z2 = taus(z[1], 2, 25, 4, 4294967288)
+    z3 = taus(z[2], 3, 11, 17, 4294967280)
+    z4 = (z[3] * np.uint64(1664525) + np.uint64(1013904223)) & m32
+    mn = np.float32(min_coord)
+    rg = np.float32(range_)
+
+    def unit(x):
+        return (x & np.uint64(0xffffff)).astype(np.float32) / np.float32(16777216.0)
+    out = np.empty((n, 4), np.float32)
+    for c, x in enumerate((z1 ^ z2, z2 ^ z3, z3 ^ z4, z1 ^ z4)):
+        out[:, c] = mn + unit(x) * rg
+    return out
+
+
 # ---------------------------------------------------------------- scalars
 def encode(x, y, z, min_coord=0.0, range_=1024.0) -> int:
     return int(lib().orc_encode(x, y, z, min_coord, range_))

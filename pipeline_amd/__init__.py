@@ -55,7 +55,7 @@ class Results(ctypes.Structure):
         ("has_leaf_left", _P), ("has_leaf_right", _P), ("left_child", _P),
         ("parent", _P), ("edge_count", _P), ("offsets", _P), ("nodes", _P),
         ("n_unique", _P), ("n_oct", _P), ("oct_start", _P),
-        ("n_seg", _I32), ("n_points", _I64),
+        ("n_seg", _I32), ("n_points", _I64), ("seg_min", _P), ("seg_range", _P),
     ]
 
 
@@ -97,6 +97,7 @@ SIGNATURES = {
     "ocp_memcpy_d2h": (_INT, [_P, _P, _P, _SZ]),
     "ocp_memset": (_INT, [_P, _P, _INT, _SZ]),
     "ocp_copy": (_INT, [_P, _P, _P, _SZ]),
+    "ocp_k_InitRandomVec4": (_INT, [_P, _P, _I32, _F, _F, _I32]),
     "ocp_k_ComputeMortonCode": (_INT, [_P, _P, _P, _SZ, _F, _F]),
     "ocp_k_ComputeMortonCode3": (_INT, [_P, _P, _P, _SZ, _F, _F]),
     "ocp_k_GlobalHistogram": (_INT, [_P, _P, _P, _U32]),
@@ -112,6 +113,7 @@ SIGNATURES = {
     "ocp_k_LinkLeafNodes": (_INT, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32]),
     "ocp_build": (_INT, [_P, _P, _INT, _I64, _F, _F]),
     "ocp_build_batched": (_INT, [_P, _P, _INT, _P, _P, _P, _I32]),
+    "ocp_build_autobox": (_INT, [_P, _P, _INT, _I64]),
     "ocp_check_batch": (_INT, [_P, _P, _P, _I32]),
     "ocp_set_graph_mode": (_INT, [_P, _INT]),
     "ocp_get_results": (_INT, [_P, ctypes.POINTER(Results)]),
@@ -304,6 +306,12 @@ class Context:
             pass
 
     # ------------------------------------------------------------ stages
+    def k_InitRandomVec4(self, data: DeviceArray, n: int, min_coord: float, range_: float,
+                         seed: int):
+        """The reference's GPU generator (kernels/cuda/00_init.cu:27-50)."""
+        self._check(self.lib.ocp_k_InitRandomVec4(self.h, data.ptr, int(n), float(min_coord),
+                                                  float(range_), int(seed)))
+
     def k_ComputeMortonCode(self, data: DeviceArray, keys: DeviceArray, n: int,
                             min_coord: float, range_: float):
         fn = (self.lib.ocp_k_ComputeMortonCode3 if data.shape[-1] == 3
@@ -394,13 +402,32 @@ class Context:
         f3 = int(points.shape[-1] == 3)
         self._check(self.lib.ocp_build(self.h, points.ptr, f3, n, min_coord, range_))
 
-    def build_batched(self, points: DeviceArray, seg_offsets, seg_min, seg_range):
+    def build_batched(self, points: DeviceArray, seg_offsets, seg_min=None, seg_range=None):
+        """seg_min/seg_range None: each cloud's cube box is reduced on the device."""
         offs = np.ascontiguousarray(seg_offsets, np.int32)
+        f3 = int(points.shape[-1] == 3)
+        if seg_min is None and seg_range is None:
+            self._check(self.lib.ocp_build_batched(self.h, points.ptr, f3, _ptr(offs), None,
+                                                   None, offs.size - 1))
+            return
         mn = np.ascontiguousarray(seg_min, np.float32)
         rg = np.ascontiguousarray(seg_range, np.float32)
-        f3 = int(points.shape[-1] == 3)
         self._check(self.lib.ocp_build_batched(self.h, points.ptr, f3, _ptr(offs),
                                                _ptr(mn), _ptr(rg), offs.size - 1))
+
+    def build_autobox(self, points: DeviceArray, n: int | None = None):
+        n = points.shape[0] if n is None else n
+        f3 = int(points.shape[-1] == 3)
+        self._check(self.lib.ocp_build_autobox(self.h, points.ptr, f3, int(n)))
+
+    def boxes(self):
+        """(seg_min, seg_range) host copies of the boxes the last batched or
+        autobox build used."""
+        r = self.results()
+        if not r.seg_min:
+            raise OcpError("the last build took its box from the caller")
+        return (d2h(self, r.seg_min, r.n_seg, np.float32), d2h(self, r.seg_range, r.n_seg,
+                                                                np.float32))
 
     def results(self) -> Results:
         r = Results()

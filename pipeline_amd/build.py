@@ -14,7 +14,7 @@ OUT_DIR = os.path.join(HERE, "_lib")
 BUILD_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libocpipe.so")
 
-SOURCES = ["k_util.hip", "k_morton.hip", "k_onesweep.hip", "k_unique.hip",
+SOURCES = ["k_util.hip", "k_init.hip", "k_morton.hip", "k_onesweep.hip", "k_unique.hip",
            "k_radix_tree.hip", "k_edge_scan.hip", "k_octree.hip", "k_octree_stage.hip",
            "k_learned.hip", "capi.hip"]
 

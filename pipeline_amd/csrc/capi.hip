@@ -52,11 +52,13 @@ struct Workspace {
   int32_t* oct_start = nullptr;  // seg_cap + 1
   int32_t* tables = nullptr;     // seg_start + 4 tile maps, 5 * (seg_cap+1)
   float* boxes = nullptr;        // seg_min[seg_cap], seg_range[seg_cap]
+  uint32_t* box_mm = nullptr;    // per segment {min, max} while boxes are reduced
 };
 
 struct Plan {
   int32_t n_seg = 1;
   int64_t n_total = 0;
+  bool device_boxes = false;  // boxes reduced on the device into ws.boxes
   SegView sv{};
   int32_t tiles_morton = 0, tiles_sort = 0, tiles_scan = 0, tiles_node = 0;
   const int32_t* ts_morton = nullptr;
@@ -234,6 +236,7 @@ void free_workspace(Workspace& w) {
   dev_free(w.hr); dev_free(w.lc); dev_free(w.par); dev_free(w.ppn); dev_free(w.edge);
   dev_free(w.offs); dev_free(w.nodes); dev_free(w.hist); dev_free(w.flags);
   dev_free(w.counts); dev_free(w.oct_start); dev_free(w.tables); dev_free(w.boxes);
+  dev_free(w.box_mm);
   w = Workspace{};
 }
 
@@ -277,13 +280,14 @@ int ensure_workspace(ocp_ctx c, int64_t n, int32_t n_seg, int64_t flag_words) {
   if (grow_seg) {
     const int32_t cap = std::max<int32_t>(n_seg, 1);
     dev_free(w.hist); dev_free(w.counts); dev_free(w.oct_start); dev_free(w.tables);
-    dev_free(w.boxes);
+    dev_free(w.boxes); dev_free(w.box_mm);
     int r = 0;
     if ((r = dev_alloc(c, &w.hist, int64_t(cap) * 1024)) ||
         (r = dev_alloc(c, &w.counts, int64_t(cap) * 2)) ||
         (r = dev_alloc(c, &w.oct_start, int64_t(cap) + 1)) ||
         (r = dev_alloc(c, &w.tables, 5 * (int64_t(cap) + 1))) ||
-        (r = dev_alloc(c, &w.boxes, 2 * int64_t(cap))))
+        (r = dev_alloc(c, &w.boxes, 2 * int64_t(cap))) ||
+        (r = dev_alloc(c, &w.box_mm, 2 * int64_t(cap))))
       return r;
     w.seg_cap = cap;
   }
@@ -339,14 +343,16 @@ Plan plain_plan(const TuneTable& tt, int64_t n, float mn, float rg) {
 int validate_batch(const int32_t* offsets, const float* smin, const float* srg,
                    int32_t n_seg, std::string* why) {
   if (n_seg < 1) return *why = "n_seg must be >= 1", OCP_ERR_INVALID;
-  if (!offsets || !smin || !srg) return *why = "null segment table", OCP_ERR_INVALID;
+  if (!offsets) return *why = "null segment table", OCP_ERR_INVALID;
+  if (!smin != !srg) return *why = "seg_min and seg_range must both be given or both be NULL",
+                     OCP_ERR_INVALID;
   if (offsets[0] != 0) return *why = "seg_offsets[0] must be 0", OCP_ERR_INVALID;
   for (int32_t k = 0; k < n_seg; ++k) {
     const int64_t len = int64_t(offsets[k + 1]) - offsets[k];
     if (len < 0) return *why = "seg_offsets must be non-decreasing", OCP_ERR_INVALID;
     if (len > kMaxSegPoints) return *why = "segment larger than 2^27 points", OCP_ERR_CAPACITY;
-    if (!(srg[k] > 0.0f)) return *why = "seg_range must be > 0", OCP_ERR_INVALID;
-    if (!(smin[k] == smin[k])) return *why = "seg_min must not be NaN", OCP_ERR_INVALID;
+    if (srg && !(srg[k] > 0.0f)) return *why = "seg_range must be > 0", OCP_ERR_INVALID;
+    if (smin && !(smin[k] == smin[k])) return *why = "seg_min must not be NaN", OCP_ERR_INVALID;
   }
   if (int64_t(offsets[n_seg]) > kMaxPoints)
     return *why = "more than 2^30-1 points", OCP_ERR_CAPACITY;
@@ -370,6 +376,7 @@ int batched_plan(ocp_ctx c, const int32_t* offsets, const float* smin,
   Plan p;
   p.n_seg = n_seg;
   p.n_total = n;
+  p.device_boxes = !smin;
   std::vector<int32_t> host(5 * static_cast<size_t>(n_seg + 1));
   int64_t tm = 0, ts = 0, tc = 0, tn = 0;
   for (int32_t k = 0; k <= n_seg; ++k) {
@@ -398,9 +405,11 @@ int batched_plan(ocp_ctx c, const int32_t* offsets, const float* smin,
     c->tab_tune_key = tune_key(tt);
     c->tab_min.clear();
   }
-  const bool same_box = c->tab_min.size() == static_cast<size_t>(n_seg) &&
-                        std::equal(smin, smin + n_seg, c->tab_min.begin()) &&
-                        std::equal(srg, srg + n_seg, c->tab_rg.begin());
+  const bool dev_boxes = !smin;
+  const bool same_box = dev_boxes || (c->tab_min.size() == static_cast<size_t>(n_seg) &&
+                                      std::equal(smin, smin + n_seg, c->tab_min.begin()) &&
+                                      std::equal(srg, srg + n_seg, c->tab_rg.begin()));
+  if (dev_boxes) c->tab_min.clear();  // ws.boxes is overwritten on the device
   if (!same_box) {
     OCP_HIP(c, hipMemcpyAsync(w.boxes, smin, sizeof(float) * n_seg, hipMemcpyHostToDevice, c->stream));
     OCP_HIP(c, hipMemcpyAsync(w.boxes + w.seg_cap, srg, sizeof(float) * n_seg,
@@ -495,6 +504,9 @@ bool enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
   { KScope k(c, kKBegin, capturing);
     launch_begin(c->ctrl, w.hist, int64_t(p.n_seg) * 1024, s,
                  reinterpret_cast<uint32_t*>(w.counts), int64_t(w.seg_cap) * 2); }
+  if (p.device_boxes)
+    launch_boxes(pts, f3, p.sv, p.ts_morton, p.tiles_morton, morton_tile(tt), w.box_mm, w.boxes,
+                 w.boxes + w.seg_cap, s);
   { KScope k(c, kKMorton, capturing);
     launch_morton(pts, f3, w.keys_a, p.sv, p.ts_morton, p.tiles_morton, w.hist, tt, s); }
   mark(c, 1, capturing);
@@ -760,6 +772,17 @@ int ocp_memset(ocp_ctx c, void* dst, int value, size_t bytes) {
 }
 
 // ---------------------------------------------------------------- stages
+int ocp_k_InitRandomVec4(ocp_ctx c, float* data, int32_t n, float mn, float rg, int32_t seed) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (n <= 0) return OCP_OK;
+  if (!data) return fail(c, OCP_ERR_INVALID, "null pointer");
+  if (reinterpret_cast<uintptr_t>(data) & 15u)
+    return fail(c, OCP_ERR_INVALID, "float4 output must be 16-byte aligned");
+  launch_init_random(data, n, mn, rg, seed, c->stream);
+  OCP_CHECK_LAUNCH(c);
+  return OCP_OK;
+}
+
 static int morton_common(ocp_ctx c, const float* data, uint32_t* keys, size_t n,
                          float mn, float rg, bool f3) {
   if (check_ctx(c)) return OCP_ERR_INVALID;
@@ -1018,6 +1041,28 @@ int ocp_build(ocp_ctx c, const float* points, int float3, int64_t n, float mn, f
   return run_plan(c, points, float3, p, graph_key("plain", points, float3, n, mn, rg, nullptr, tt));
 }
 
+int ocp_build_autobox(ocp_ctx c, const float* points, int float3, int64_t n) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (n < 0) return fail(c, OCP_ERR_INVALID, "n < 0");
+  if (n > 0 && !points) return fail(c, OCP_ERR_INVALID, "null points");
+  if (n > kMaxSegPoints) return fail(c, OCP_ERR_CAPACITY, "a single cloud holds at most 2^27 points");
+  if (!float3 && (reinterpret_cast<uintptr_t>(points) & 15u))
+    return fail(c, OCP_ERR_INVALID, "float4 input must be 16-byte aligned");
+  const TuneTable& tt = c->tune;
+  if (int r = ensure_workspace(c, n, 1, flag_words_for(tt, n, 1))) return r;
+  Plan p = plain_plan(tt, n, 0.0f, 1.0f);
+  p.device_boxes = true;
+  p.sv.mn = c->ws.boxes;
+  p.sv.rg = c->ws.boxes + c->ws.seg_cap;
+  c->last.valid = true;
+  c->last.batched = false;
+  c->last.pts = points;
+  c->last.float3 = float3;
+  c->last.n = n;
+  return run_plan(c, points, float3, p,
+                  graph_key("plain-autobox", points, float3, n, 0, 0, nullptr, tt));
+}
+
 int ocp_build_batched(ocp_ctx c, const float* points, int float3, const int32_t* offsets,
                       const float* smin, const float* srg, int32_t n_seg) {
   if (check_ctx(c)) return OCP_ERR_INVALID;
@@ -1031,10 +1076,16 @@ int ocp_build_batched(ocp_ctx c, const float* points, int float3, const int32_t*
   c->last.pts = points;
   c->last.float3 = float3;
   c->last.offsets.assign(offsets, offsets + n_seg + 1);
-  c->last.seg_min.assign(smin, smin + n_seg);
-  c->last.seg_range.assign(srg, srg + n_seg);
+  if (smin) {
+    c->last.seg_min.assign(smin, smin + n_seg);
+    c->last.seg_range.assign(srg, srg + n_seg);
+  } else {
+    c->last.seg_min.clear();
+    c->last.seg_range.clear();
+  }
   return run_plan(c, points, float3, p,
-                  graph_key("batch", points, float3, p.n_total, 0, 0, &c->last.offsets, c->tune));
+                  graph_key(smin ? "batch" : "batch-autobox", points, float3, p.n_total, 0, 0,
+                            &c->last.offsets, c->tune));
 }
 
 int ocp_check_batch(const int32_t* offsets, const float* smin, const float* srg,
@@ -1068,6 +1119,9 @@ int ocp_get_results(ocp_ctx c, ocp_results* out) {
   out->oct_start = w.oct_start;
   out->n_seg = c->last_plan.n_seg;
   out->n_points = c->last_plan.n_total;
+  const bool boxes = c->last_plan.device_boxes || c->last_plan.sv.start;
+  out->seg_min = boxes ? w.boxes : nullptr;
+  out->seg_range = boxes ? w.boxes + w.seg_cap : nullptr;
   return OCP_OK;
 }
 

@@ -96,6 +96,15 @@ void launch_oct_offsets(const int32_t* kalloc, int32_t n_seg,
 void launch_zero_nodes(OctNode* nodes, const uint32_t* ctrl, int64_t k_cap,
                        hipStream_t s);
 
+// ---- input side (k_init.hip)
+// n float4 points from the reference's hybrid Tausworthe generator.
+void launch_init_random(float* out, int32_t n, float mn, float rg, int32_t seed, hipStream_t s);
+// Per-segment cube boxes into mn[n_seg], rg[n_seg]; mm: 2 * n_seg words of
+// scratch. Tiles of `tile` points per the tile map.
+void launch_boxes(const float* pts, bool float3, const SegView& sv, const int32_t* tile_start,
+                  int32_t total_tiles, int tile, uint32_t* mm, float* mn, float* rg,
+                  hipStream_t s);
+
 // ---- stage 1 (k_morton.hip)
 void launch_morton(const float* pts, bool float3, uint32_t* keys,
                    const SegView& sv, const int32_t* tile_start,
