@@ -138,12 +138,24 @@ int ocp_k_SortKeys(ocp_ctx ctx, uint32_t* keys, uint32_t* keys_alt,
 int ocp_k_SortKeys_static(ocp_ctx ctx, uint32_t* keys, uint32_t* keys_alt,
                           uint32_t n);
 
+/* The key->point permutation, which the reference does not keep (it sorts
+ * bare keys, kernels/cuda/02_sort.cu:566-574, and uniques in place,
+ * kernels/03_unique.cpp:2-16): a stable sort of n keys carrying one 32-bit
+ * value each (histogram + 4 stable passes; keys/vals -> alt -> keys/vals).
+ * With vals = 0..n-1 the values end as the stable argsort of the keys. */
+int ocp_k_SortPairs(ocp_ctx ctx, uint32_t* keys, uint32_t* vals,
+                    uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n);
+
 /* ------------------------------------------------------------ stage 3 */
 /* Replaces gpu::k_CountUnique (include/cuda/kernels/03_unique.cuh:5) and
  * k_CountUnique (include/kernels/03_unique.hpp:4): in-place unique of n
  * sorted keys; the count is written to num_unique_out (device int32). */
 int ocp_k_CountUnique(ocp_ctx ctx, uint32_t* keys, int32_t* num_unique_out,
                       int32_t n);
+/* The same with a value per key: each kept key keeps the value of the first
+ * key of its run (after ocp_k_SortPairs, its smallest point index). */
+int ocp_k_CountUniquePairs(ocp_ctx ctx, uint32_t* keys, uint32_t* vals,
+                           int32_t* num_unique_out, int32_t n);
 
 /* ------------------------------------------------------------ stage 4 */
 /* Replaces gpu::k_BuildRadixTree (include/cuda/kernels/04_radix_tree.cuh:7-13)
@@ -224,6 +236,11 @@ int ocp_check_batch(const int32_t* seg_offsets, const float* seg_min,
  * build. The reference fixes the box at [0, 1024) (benchmarks/config.hpp:
  * 7-10). */
 int ocp_build_autobox(ocp_ctx ctx, const float* points, int float3, int64_t n);
+/* Builds carry the key->point permutation when enabled: every sort pass
+ * moves a point index with its key (and is stable), unique keeps each key's
+ * first point, and ocp_results gives both (sorted_index, unique_index).
+ * Indices are relative to each cloud's first point. Off by default. */
+int ocp_set_permutation(ocp_ctx ctx, int enable);
 /* Capture each distinct build shape once into a hipGraph and replay it. */
 int ocp_set_graph_mode(ocp_ctx ctx, int enable);
 
@@ -247,6 +264,9 @@ typedef struct ocp_results {
   int64_t n_points;
   const float* seg_min;           /* device, n_seg: the boxes a batched or */
   const float* seg_range;         /* autobox build used (NULL otherwise)   */
+  const uint32_t* sorted_index;   /* n: point of each sorted key, and the  */
+  const uint32_t* unique_index;   /* first point of each unique key (NULL  */
+                                  /* unless ocp_set_permutation is on)     */
 } ocp_results;
 int ocp_get_results(ocp_ctx ctx, ocp_results* out);
 /* Synchronises and copies the per-segment U and K (host arrays of n_seg). */

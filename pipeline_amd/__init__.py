@@ -56,6 +56,7 @@ class Results(ctypes.Structure):
         ("parent", _P), ("edge_count", _P), ("offsets", _P), ("nodes", _P),
         ("n_unique", _P), ("n_oct", _P), ("oct_start", _P),
         ("n_seg", _I32), ("n_points", _I64), ("seg_min", _P), ("seg_range", _P),
+        ("sorted_index", _P), ("unique_index", _P),
     ]
 
 
@@ -103,6 +104,9 @@ SIGNATURES = {
     "ocp_k_GlobalHistogram": (_INT, [_P, _P, _P, _U32]),
     "ocp_k_DigitBinning": (_INT, [_P, _P, _P, _P, _U32, _U32]),
     "ocp_k_SortKeys": (_INT, [_P, _P, _P, _U32]),
+    "ocp_k_SortPairs": (_INT, [_P, _P, _P, _P, _P, _U32]),
+    "ocp_k_CountUniquePairs": (_INT, [_P, _P, _P, _P, _I32]),
+    "ocp_set_permutation": (_INT, [_P, _INT]),
     "ocp_k_SortKeys_static": (_INT, [_P, _P, _P, _U32]),
     "ocp_k_CountUnique": (_INT, [_P, _P, _P, _I32]),
     "ocp_k_BuildRadixTree": (_INT, [_P, _I32, _P, _P, _P, _P, _P, _P]),
@@ -237,6 +241,8 @@ class BuildOutput:
     nodes: list
     n_unique: np.ndarray
     n_oct: np.ndarray
+    sorted_index: list | None = None   # with the permutation on
+    unique_index: list | None = None
 
 
 class Context:
@@ -334,6 +340,21 @@ class Context:
         """ocp_k_SortKeys with its scratch in the library's fixed device array
         (no allocation; at most STATIC_SORT_MAX_KEYS keys)."""
         self._check(self.lib.ocp_k_SortKeys_static(self.h, keys.ptr, keys_alt.ptr, n))
+
+    def k_SortPairs(self, keys, vals, keys_alt, vals_alt, n: int):
+        self._check(self.lib.ocp_k_SortPairs(self.h, keys.ptr, vals.ptr, keys_alt.ptr,
+                                             vals_alt.ptr, n))
+
+    def k_CountUniquePairs(self, keys, vals, n: int) -> int:
+        out = self.empty(1, np.int32)
+        self._check(self.lib.ocp_k_CountUniquePairs(self.h, keys.ptr, vals.ptr, out.ptr, n))
+        u = int(out.to_host()[0])
+        out.free()
+        self._owned.remove(out)
+        return u
+
+    def set_permutation(self, enable: bool):
+        self._check(self.lib.ocp_set_permutation(self.h, int(bool(enable))))
 
     def k_CountUnique(self, keys: DeviceArray, n: int) -> int:
         out = self.empty(1, np.int32)
@@ -460,6 +481,11 @@ class Context:
         ost = d2h(self, r.oct_start, n_seg + 1, np.int32)
         nodes = d2h(self, r.nodes, int(ost[-1]) * 60, np.uint8).view(OCT_NODE_DTYPE)
         out = BuildOutput([], [], [], [], [], [], [], [], [], [], u, k)
+        si = ui = None
+        if r.sorted_index:
+            si = d2h(self, r.sorted_index, total, np.uint32)
+            ui = d2h(self, r.unique_index, total, np.uint32)
+            out.sorted_index, out.unique_index = [], []
         for s in range(n_seg):
             a, b = int(offs[s]), int(offs[s + 1])
             m = max(int(u[s]) - 1, 0)
@@ -473,6 +499,9 @@ class Context:
             out.edge_count.append(e[a:a + m])
             out.offsets.append(p[a:a + m])
             out.nodes.append(nodes[int(ost[s]):int(ost[s]) + int(k[s])])
+            if si is not None:
+                out.sorted_index.append(si[a:b])
+                out.unique_index.append(ui[a:a + int(u[s])])
         return out
 
     def time_builds(self, points: DeviceArray, warmup: int, steps: int,

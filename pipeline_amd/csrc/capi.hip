@@ -53,12 +53,16 @@ struct Workspace {
   int32_t* tables = nullptr;     // seg_start + 4 tile maps, 5 * (seg_cap+1)
   float* boxes = nullptr;        // seg_min[seg_cap], seg_range[seg_cap]
   uint32_t* box_mm = nullptr;    // per segment {min, max} while boxes are reduced
+  uint32_t* vals_a = nullptr;    // key->point permutation (allocated when enabled): sorted
+  uint32_t* vals_b = nullptr;    //   point indices, then each unique key's first point
+  int64_t vals_cap = 0;
 };
 
 struct Plan {
   int32_t n_seg = 1;
   int64_t n_total = 0;
   bool device_boxes = false;  // boxes reduced on the device into ws.boxes
+  bool permutation = false;   // point indices carried through sort and unique
   SegView sv{};
   int32_t tiles_morton = 0, tiles_sort = 0, tiles_scan = 0, tiles_node = 0;
   const int32_t* ts_morton = nullptr;
@@ -175,6 +179,7 @@ struct ocp_context {
   bool have_result = false;
   Plan last_plan;
   bool graph_mode = false;
+  bool permutation = false;
   std::map<std::string, hipGraphExec_t> graphs;
   bool stage_timing = false;
   hipEvent_t ev[8] = {};
@@ -236,7 +241,7 @@ void free_workspace(Workspace& w) {
   dev_free(w.hr); dev_free(w.lc); dev_free(w.par); dev_free(w.ppn); dev_free(w.edge);
   dev_free(w.offs); dev_free(w.nodes); dev_free(w.hist); dev_free(w.flags);
   dev_free(w.counts); dev_free(w.oct_start); dev_free(w.tables); dev_free(w.boxes);
-  dev_free(w.box_mm);
+  dev_free(w.box_mm); dev_free(w.vals_a); dev_free(w.vals_b);
   w = Workspace{};
 }
 
@@ -298,6 +303,21 @@ int ensure_workspace(ocp_ctx c, int64_t n, int32_t n_seg, int64_t flag_words) {
     OCP_HIP(c, hipMemsetAsync(w.flags, 0, sizeof(uint64_t) * cap, c->stream));
     w.flag_cap = cap;
   }
+  return OCP_OK;
+}
+
+// The permutation buffers follow the workspace's point capacity.
+int ensure_perm(ocp_ctx c) {
+  Workspace& w = c->ws;
+  if (!c->permutation || w.vals_cap >= w.n_cap) return OCP_OK;
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  clear_graphs(c);
+  dev_free(w.vals_a);
+  dev_free(w.vals_b);
+  w.vals_cap = 0;
+  int r = 0;
+  if ((r = dev_alloc(c, &w.vals_a, w.n_cap)) || (r = dev_alloc(c, &w.vals_b, w.n_cap))) return r;
+  w.vals_cap = w.n_cap;
   return OCP_OK;
 }
 
@@ -512,17 +532,29 @@ bool enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
   mark(c, 1, capturing);
   uint32_t* src = w.keys_a;
   uint32_t* dst = w.keys_b;
+  uint32_t* vsrc = nullptr;  // the first pass numbers the points itself
+  uint32_t* vdst = w.vals_b;
   for (int pass = 0; pass < 4; ++pass) {
     KScope k(c, kKPass0 + pass, capturing);
-    // The first pass may reorder equal digits: the keys carry no payload.
-    launch_onesweep_pass(src, dst, p.sv, p.ts_sort, p.tiles_sort, w.hist, w.flags,
-                         c->ctrl, pass, pass != 0, tt, s);
+    if (p.permutation) {
+      // Point indices ride with the keys; every pass is stable, so they end
+      // as the stable argsort of the keys.
+      launch_onesweep_pass_kv(src, dst, vsrc, vdst, p.sv, p.ts_sort, p.tiles_sort, w.hist,
+                              w.flags, c->ctrl, pass, tt, s);
+      vsrc = vdst;
+      vdst = (vdst == w.vals_b) ? w.vals_a : w.vals_b;
+    } else {
+      // The first pass may reorder equal digits: the keys carry no payload.
+      launch_onesweep_pass(src, dst, p.sv, p.ts_sort, p.tiles_sort, w.hist, w.flags,
+                           c->ctrl, pass, pass != 0, tt, s);
+    }
     std::swap(src, dst);
   }
   mark(c, 2, capturing);
   { KScope k(c, kKUnique, capturing);
     launch_unique(w.keys_a, w.keys_b, p.sv, p.ts_scan, p.tiles_scan, w.flags, c->ctrl,
-                  n_unique, kalloc, tt, s); }
+                  n_unique, kalloc, tt, s, p.permutation ? w.vals_a : nullptr,
+                  p.permutation ? w.vals_b : nullptr); }
   mark(c, 3, capturing);
   { KScope k(c, kKRadixTree, capturing);
     launch_radix_tree(w.keys_b, w.prefix_n, w.hl, w.hr, w.lc, w.par, w.ppn, p.sv, p.ts_node,
@@ -554,7 +586,8 @@ std::string graph_key(const char* kind, const float* pts, int f3, int64_t n,
 }
 
 int run_plan(ocp_ctx c, const float* pts, int f3, const Plan& p,
-             const std::string& key) {
+             const std::string& key_in) {
+  const std::string key = key_in + (p.permutation ? "|perm" : "");
   if (c->graph_mode) {
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
@@ -888,6 +921,52 @@ int ocp_k_SortKeys_static(ocp_ctx c, uint32_t* keys, uint32_t* alt, uint32_t n) 
   return OCP_OK;
 }
 
+int ocp_k_SortPairs(ocp_ctx c, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
+                    uint32_t* vals_alt, uint32_t n) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (n <= 1) return OCP_OK;
+  if (!keys || !vals || !keys_alt || !vals_alt) return fail(c, OCP_ERR_INVALID, "null pointer");
+  if (n > kMaxPoints) return fail(c, OCP_ERR_CAPACITY, "n > 2^30-1");
+  const int32_t tiles = static_cast<int32_t>(cdiv(n, sort_tile(c->tune)));
+  if (int r = ensure_scratch(c, int64_t(tiles) * 256)) return r;
+  const SegView sv = single_view(n, 0.0f, 1.0f);
+  launch_begin(c->ctrl, c->s_hist, 1024, c->stream);
+  launch_histogram(keys, sv, nullptr, static_cast<int32_t>(cdiv(n, morton_tile(c->tune))),
+                   c->s_hist, c->tune, c->stream);
+  uint32_t* src = keys;
+  uint32_t* dst = keys_alt;
+  uint32_t* vs = vals;
+  uint32_t* vd = vals_alt;
+  for (int pass = 0; pass < 4; ++pass) {
+    launch_onesweep_pass_kv(src, dst, vs, vd, sv, nullptr, tiles, c->s_hist, c->s_flags, c->ctrl,
+                            pass, c->tune, c->stream);
+    std::swap(src, dst);
+    std::swap(vs, vd);
+  }
+  OCP_CHECK_LAUNCH(c);
+  return OCP_OK;
+}
+
+int ocp_k_CountUniquePairs(ocp_ctx c, uint32_t* keys, uint32_t* vals, int32_t* num_unique_out,
+                           int32_t n) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (!num_unique_out || (n > 0 && (!keys || !vals))) return fail(c, OCP_ERR_INVALID, "null pointer");
+  if (n < 0) return fail(c, OCP_ERR_INVALID, "n < 0");
+  if (n > kMaxPoints) return fail(c, OCP_ERR_CAPACITY, "n > 2^30-1");
+  launch_fill_u32(reinterpret_cast<uint32_t*>(num_unique_out), 0u, 1, c->stream);
+  if (n == 0) {
+    OCP_CHECK_LAUNCH(c);
+    return OCP_OK;
+  }
+  const int32_t tiles = static_cast<int32_t>(cdiv(n, scan_tile(c->tune)));
+  if (int r = ensure_scratch(c, 2 * int64_t(tiles))) return r;
+  launch_begin(c->ctrl, nullptr, 0, c->stream);
+  launch_unique(keys, keys, single_view(n, 0.0f, 1.0f), nullptr, tiles, c->s_flags, c->ctrl,
+                num_unique_out, nullptr, c->tune, c->stream, vals, vals);
+  OCP_CHECK_LAUNCH(c);
+  return OCP_OK;
+}
+
 int ocp_k_CountUnique(ocp_ctx c, uint32_t* keys, int32_t* num_unique_out, int32_t n) {
   if (check_ctx(c)) return OCP_ERR_INVALID;
   if (!num_unique_out || (n > 0 && !keys)) return fail(c, OCP_ERR_INVALID, "null pointer");
@@ -1030,7 +1109,9 @@ int ocp_build(ocp_ctx c, const float* points, int float3, int64_t n, float mn, f
     return fail(c, OCP_ERR_INVALID, "float4 input must be 16-byte aligned");
   const TuneTable& tt = c->tune;
   if (int r = ensure_workspace(c, n, 1, flag_words_for(tt, n, 1))) return r;
-  const Plan p = plain_plan(tt, n, mn, rg);
+  Plan p = plain_plan(tt, n, mn, rg);
+  p.permutation = c->permutation;
+  if (int r = ensure_perm(c)) return r;
   c->last.valid = true;
   c->last.batched = false;
   c->last.pts = points;
@@ -1052,6 +1133,8 @@ int ocp_build_autobox(ocp_ctx c, const float* points, int float3, int64_t n) {
   if (int r = ensure_workspace(c, n, 1, flag_words_for(tt, n, 1))) return r;
   Plan p = plain_plan(tt, n, 0.0f, 1.0f);
   p.device_boxes = true;
+  p.permutation = c->permutation;
+  if (int r = ensure_perm(c)) return r;
   p.sv.mn = c->ws.boxes;
   p.sv.rg = c->ws.boxes + c->ws.seg_cap;
   c->last.valid = true;
@@ -1069,6 +1152,8 @@ int ocp_build_batched(ocp_ctx c, const float* points, int float3, const int32_t*
   Plan p;
   if (int r = batched_plan(c, offsets, smin, srg, n_seg, &p)) return r;
   if (p.n_total > 0 && !points) return fail(c, OCP_ERR_INVALID, "null points");
+  p.permutation = c->permutation;
+  if (int r = ensure_perm(c)) return r;
   if (!float3 && (reinterpret_cast<uintptr_t>(points) & 15u))
     return fail(c, OCP_ERR_INVALID, "float4 input must be 16-byte aligned");
   c->last.valid = true;
@@ -1092,6 +1177,12 @@ int ocp_check_batch(const int32_t* offsets, const float* smin, const float* srg,
                     int32_t n_seg) {
   std::string why;
   return validate_batch(offsets, smin, srg, n_seg, &why);
+}
+
+int ocp_set_permutation(ocp_ctx c, int enable) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  c->permutation = enable != 0;
+  return OCP_OK;
 }
 
 int ocp_set_graph_mode(ocp_ctx c, int enable) {
@@ -1122,6 +1213,8 @@ int ocp_get_results(ocp_ctx c, ocp_results* out) {
   const bool boxes = c->last_plan.device_boxes || c->last_plan.sv.start;
   out->seg_min = boxes ? w.boxes : nullptr;
   out->seg_range = boxes ? w.boxes + w.seg_cap : nullptr;
+  out->sorted_index = c->last_plan.permutation ? w.vals_a : nullptr;
+  out->unique_index = c->last_plan.permutation ? w.vals_b : nullptr;
   return OCP_OK;
 }
 

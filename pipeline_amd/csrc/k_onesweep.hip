@@ -90,7 +90,40 @@ void launch_cfg(const uint32_t* in, uint32_t* out, const SegView& sv,
         in, out, sv, tile_start, nullptr, hist, pass, flags, ctrl, stage, dig);
 }
 
+template <int WAVES, int KPT>
+void launch_cfg_kv(const uint32_t* in, uint32_t* out, const uint32_t* vin, uint32_t* vout,
+                   const SegView& sv, const int32_t* tile_start, int32_t total_tiles,
+                   const uint32_t* hist, uint64_t* flags, uint32_t* ctrl, int pass,
+                   hipStream_t s) {
+  const uint32_t stage = kStagePass0 + static_cast<uint32_t>(pass);
+  const RadixDigit dig{8u * static_cast<uint32_t>(pass)};
+  onesweep_kernel<WAVES, KPT, RadixDigit, true, true><<<total_tiles, WAVES * 64, 0, s>>>(
+      in, out, sv, tile_start, nullptr, hist, pass, flags, ctrl, stage, dig, vin, vout);
+}
+
 }  // namespace
+
+void launch_onesweep_pass_kv(const uint32_t* in, uint32_t* out, const uint32_t* vin,
+                             uint32_t* vout, const SegView& sv, const int32_t* tile_start,
+                             int32_t total_tiles, const uint32_t* hist, uint64_t* flags,
+                             uint32_t* ctrl, int pass, const TuneTable& tt, hipStream_t s) {
+  if (total_tiles <= 0) return;
+#define OCP_SORT_KV_CFG(W, K)                                                         \
+  if (tt.sort_waves == W && tt.sort_kpt == K) {                                       \
+    launch_cfg_kv<W, K>(in, out, vin, vout, sv, tile_start, total_tiles, hist, flags, \
+                        ctrl, pass, s);                                               \
+    return;                                                                           \
+  }
+  OCP_SORT_KV_CFG(4, 16)
+  OCP_SORT_KV_CFG(4, 24)
+  OCP_SORT_KV_CFG(8, 8)
+  OCP_SORT_KV_CFG(8, 12)
+  OCP_SORT_KV_CFG(8, 16)
+  OCP_SORT_KV_CFG(16, 8)
+#undef OCP_SORT_KV_CFG
+  launch_cfg_kv<8, 15>(in, out, vin, vout, sv, tile_start, total_tiles, hist, flags, ctrl,
+                       pass, s);
+}
 
 bool sort_config_supported(int waves, int kpt) {
   return (waves == 4 && (kpt == 16 || kpt == 24)) ||

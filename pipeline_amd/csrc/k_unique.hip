@@ -11,6 +11,10 @@
 // consecutive output slots. The segment's last tile
 // writes U and seeds the octree slot count (1 when U >= 1).
 //
+// With values (the key->point permutation), each kept key's value - the
+// first of its run, i.e. the smallest point index after a stable sort - is
+// compacted beside it, in place or not.
+//
 // In place (out == in) is safe: a tile reads all its keys, and its
 // predecessor key, before publishing its count, a tile only writes below
 // its own end, and it can only reach the predecessor key of the next tile
@@ -21,16 +25,17 @@
 namespace ocp {
 namespace {
 
-template <int IPT>
+template <int IPT, bool KV>
 __global__ __launch_bounds__(256) void unique_kernel(
     const uint32_t* in, uint32_t* out, SegView sv,
     const int32_t* __restrict__ tile_start, uint64_t* __restrict__ flags,
     uint32_t* __restrict__ ctrl, int32_t* __restrict__ n_unique,
-    int32_t* __restrict__ kalloc) {
+    int32_t* __restrict__ kalloc, const uint32_t* vin, uint32_t* vout) {
   constexpr int THREADS = 256;
   constexpr int NW = THREADS / 64;
   constexpr int T = THREADS * IPT;
   __shared__ uint32_t s_keys[T];
+  __shared__ uint32_t s_vals[KV ? T : 1];
   __shared__ uint32_t s_rows[IPT * NW];
   __shared__ uint64_t s_mask[IPT * NW];
   __shared__ uint32_t s_excl;
@@ -51,6 +56,7 @@ __global__ __launch_bounds__(256) void unique_kernel(
   // All loads are issued before any is used (a load guarded and consumed
   // in the same iteration keeps one load in flight per wave).
   uint32_t keys[IPT];
+  uint32_t vals[KV ? IPT : 1];
   const uint32_t* src = in + static_cast<int64_t>(base) + tid;
   if (cnt == T) {
 #pragma unroll
@@ -58,6 +64,11 @@ __global__ __launch_bounds__(256) void unique_kernel(
   } else {
 #pragma unroll
     for (int k = 0; k < IPT; ++k) keys[k] = k * THREADS + tid < cnt ? src[k * THREADS] : 0u;
+  }
+  if constexpr (KV) {
+    const uint32_t* vsrc = vin + static_cast<int64_t>(base) + tid;
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) vals[k] = k * THREADS + tid < cnt ? vsrc[k * THREADS] : 0u;
   }
 #pragma unroll
   for (int k = 0; k < IPT; ++k) s_keys[k * THREADS + tid] = keys[k];
@@ -116,25 +127,42 @@ __global__ __launch_bounds__(256) void unique_kernel(
     rb[k] = s_rows[k * NW + w];
   }
 #pragma unroll
-  for (int k = 0; k < IPT; ++k)
-    if ((mk[k] >> lane) & 1ull) s_keys[rb[k] + static_cast<uint32_t>(__popcll(mk[k] & lt))] = keys[k];
+  for (int k = 0; k < IPT; ++k) {
+    if ((mk[k] >> lane) & 1ull) {
+      const uint32_t pos = rb[k] + static_cast<uint32_t>(__popcll(mk[k] & lt));
+      s_keys[pos] = keys[k];
+      if constexpr (KV) s_vals[pos] = vals[k];
+    }
+  }
   __syncthreads();
   uint32_t* dst = out + sb + excl;
   uint32_t kv[IPT];
+  uint32_t vv[KV ? IPT : 1];
 #pragma unroll
-  for (int k = 0; k < IPT; ++k) kv[k] = s_keys[k * THREADS + tid];
+  for (int k = 0; k < IPT; ++k) {
+    kv[k] = s_keys[k * THREADS + tid];
+    if constexpr (KV) vv[k] = s_vals[k * THREADS + tid];
+  }
 #pragma unroll
-  for (int k = 0; k < IPT; ++k)
-    if (static_cast<uint32_t>(k * THREADS + tid) < total) dst[k * THREADS + tid] = kv[k];
+  for (int k = 0; k < IPT; ++k) {
+    if (static_cast<uint32_t>(k * THREADS + tid) < total) {
+      dst[k * THREADS + tid] = kv[k];
+      if constexpr (KV) vout[sb + excl + k * THREADS + tid] = vv[k];
+    }
+  }
 }
 
 template <int IPT>
 void launch_ipt(const uint32_t* in, uint32_t* out, const SegView& sv,
                 const int32_t* tile_start, int32_t total_tiles,
                 uint64_t* flags, uint32_t* ctrl, int32_t* n_unique,
-                int32_t* kalloc, hipStream_t s) {
-  unique_kernel<IPT><<<total_tiles, 256, 0, s>>>(in, out, sv, tile_start,
-                                                 flags, ctrl, n_unique, kalloc);
+                int32_t* kalloc, hipStream_t s, const uint32_t* vin, uint32_t* vout) {
+  if (vin && vout)
+    unique_kernel<IPT, true><<<total_tiles, 256, 0, s>>>(in, out, sv, tile_start, flags, ctrl,
+                                                         n_unique, kalloc, vin, vout);
+  else
+    unique_kernel<IPT, false><<<total_tiles, 256, 0, s>>>(in, out, sv, tile_start, flags, ctrl,
+                                                          n_unique, kalloc, nullptr, nullptr);
 }
 
 }  // namespace
@@ -144,12 +172,13 @@ bool scan_config_supported(int ipt) { return ipt == 8 || ipt == 16 || ipt == 32;
 void launch_unique(const uint32_t* in, uint32_t* out, const SegView& sv,
                    const int32_t* tile_start, int32_t total_tiles,
                    uint64_t* flags, uint32_t* ctrl, int32_t* n_unique,
-                   int32_t* kalloc, const TuneTable& tt, hipStream_t s) {
+                   int32_t* kalloc, const TuneTable& tt, hipStream_t s,
+                   const uint32_t* vin, uint32_t* vout) {
   if (total_tiles <= 0) return;
   switch (tt.scan_ipt) {
-    case 8: launch_ipt<8>(in, out, sv, tile_start, total_tiles, flags, ctrl, n_unique, kalloc, s); break;
-    case 32: launch_ipt<32>(in, out, sv, tile_start, total_tiles, flags, ctrl, n_unique, kalloc, s); break;
-    default: launch_ipt<16>(in, out, sv, tile_start, total_tiles, flags, ctrl, n_unique, kalloc, s); break;
+    case 8: launch_ipt<8>(in, out, sv, tile_start, total_tiles, flags, ctrl, n_unique, kalloc, s, vin, vout); break;
+    case 32: launch_ipt<32>(in, out, sv, tile_start, total_tiles, flags, ctrl, n_unique, kalloc, s, vin, vout); break;
+    default: launch_ipt<16>(in, out, sv, tile_start, total_tiles, flags, ctrl, n_unique, kalloc, s, vin, vout); break;
   }
 }
 

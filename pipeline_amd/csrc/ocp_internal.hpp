@@ -128,11 +128,21 @@ void launch_onesweep_pass(const uint32_t* in, uint32_t* out,
                           uint64_t* flags, uint32_t* ctrl, int pass, bool stable,
                           const TuneTable& tt, hipStream_t s);
 
+// Stable pass carrying a value with every key (vin == nullptr: the key's
+// index in its segment).
+void launch_onesweep_pass_kv(const uint32_t* in, uint32_t* out, const uint32_t* vin,
+                             uint32_t* vout, const SegView& sv, const int32_t* tile_start,
+                             int32_t total_tiles, const uint32_t* hist, uint64_t* flags,
+                             uint32_t* ctrl, int pass, const TuneTable& tt, hipStream_t s);
+
 // ---- stage 3 (k_unique.hip)
+// vin/vout (optional): each kept key's value (the first of its run) is
+// compacted beside it.
 void launch_unique(const uint32_t* in, uint32_t* out, const SegView& sv,
                    const int32_t* tile_start, int32_t total_tiles,
                    uint64_t* flags, uint32_t* ctrl, int32_t* n_unique,
-                   int32_t* kalloc, const TuneTable& tt, hipStream_t s);
+                   int32_t* kalloc, const TuneTable& tt, hipStream_t s,
+                   const uint32_t* vin = nullptr, uint32_t* vout = nullptr);
 
 // ---- stage 4 (k_radix_tree.hip)
 // n_brt: per-segment node counts; when n_unique != nullptr the count of

@@ -21,6 +21,11 @@
 // Segments sort independently: each has its own digit histogram and its own
 // lookback chain, and no tile holds keys of two segments.
 //
+// KV = true carries a 32-bit value with every key (the key->point
+// permutation): values are loaded, staged in LDS and written beside their
+// keys; with vin == nullptr a value is its key's index in the segment. Such
+// passes are always stable, so that the values end as the stable argsort.
+//
 // STABLE = false is for passes whose order among equal digits is free: the
 // first pass of an LSD sort of bare keys (its input order is arbitrary, so
 // the sorted result is the same) and scatters whose buckets are sorted
@@ -58,17 +63,20 @@ struct RadixDigit {
 // hist: per segment 1024 words; this pass reads words [slot*256, slot*256+256)
 // of its segment. tiles_dev (optional): the number of real tiles, on device,
 // when the grid is sized for an upper bound.
-template <int WAVES, int KPT, class Digit, bool STABLE = true>
+template <int WAVES, int KPT, class Digit, bool STABLE = true, bool KV = false>
 __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
     const uint32_t* __restrict__ in, uint32_t* __restrict__ out, SegView sv,
     const int32_t* __restrict__ tile_start, const int32_t* __restrict__ tiles_dev,
     const uint32_t* __restrict__ hist, int hist_slot, uint64_t* __restrict__ flags,
-    uint32_t* __restrict__ ctrl, uint32_t stage, Digit dig) {
+    uint32_t* __restrict__ ctrl, uint32_t stage, Digit dig,
+    const uint32_t* __restrict__ vin = nullptr, uint32_t* __restrict__ vout = nullptr) {
   constexpr int THREADS = WAVES * 64;
   constexpr int T = THREADS * KPT;
   static_assert(THREADS >= 256, "one thread per digit is required");
+  static_assert(!KV || STABLE, "a pass that carries values is stable");
   constexpr bool CHEAP = Digit::kCheap;
   __shared__ uint32_t s_keys[T];
+  __shared__ uint32_t s_vals[KV ? T : 1];
   __shared__ uint8_t s_dig[CHEAP ? 1 : T];
   __shared__ uint32_t s_whist[WAVES * 256];
   __shared__ int32_t s_dst[256];
@@ -96,7 +104,20 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
   uint32_t keys[KPT];
   uint32_t dg[KPT];
   uint32_t off[KPT];
-  const uint32_t* src = in + static_cast<int32_t>(w) * (64 * KPT) + static_cast<int32_t>(lane) + base;
+  uint32_t vals[KV ? KPT : 1];
+  const int32_t lbase = static_cast<int32_t>(w) * (64 * KPT) + static_cast<int32_t>(lane);
+  const uint32_t* src = in + lbase + base;
+  if constexpr (KV) {
+    // Values: given, or the key's index in its segment (the first pass of a
+    // build, where the keys are in point order).
+    const uint32_t* vsrc = vin ? vin + lbase + base : nullptr;
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      const int32_t idx = lbase + k * 64;
+      vals[k] = vsrc ? (idx < cnt ? vsrc[k * 64] : 0u)
+                     : static_cast<uint32_t>(base - sb + idx);
+    }
+  }
   if (cnt == T) {  // every tile but a segment's last: no per-key guards
 #pragma unroll
     for (int k = 0; k < KPT; ++k) keys[k] = src[k * 64];
@@ -185,6 +206,7 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
   for (int k = 0; k < KPT; ++k) {
     const uint32_t pos = wh[dg[k]] + off[k];
     s_keys[pos] = keys[k];
+    if constexpr (KV) s_vals[pos] = vals[k];
     if constexpr (!CHEAP) s_dig[pos] = static_cast<uint8_t>(dg[k]);
   }
   __syncthreads();
@@ -193,10 +215,12 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
     // All LDS reads first, then the stores (a guarded store per iteration
     // waits on its two dependent LDS reads before the next begins).
     uint32_t kv[KPT];
+    uint32_t vv[KV ? KPT : 1];
     int32_t dst[KPT];
 #pragma unroll
     for (int k = 0; k < KPT; ++k) {
       kv[k] = s_keys[k * THREADS + tid];
+      if constexpr (KV) vv[k] = s_vals[k * THREADS + tid];
       if constexpr (!CHEAP) dst[k] = s_dig[k * THREADS + tid];
     }
 #pragma unroll
@@ -205,7 +229,10 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
       dst[k] = s_dst[dst[k]] + k * THREADS + tid;
     }
 #pragma unroll
-    for (int k = 0; k < KPT; ++k) out[dst[k]] = kv[k];
+    for (int k = 0; k < KPT; ++k) {
+      out[dst[k]] = kv[k];
+      if constexpr (KV) vout[dst[k]] = vv[k];
+    }
   } else {
 #pragma unroll
     for (int k = 0; k < KPT; ++k) {
@@ -214,6 +241,7 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
         const uint32_t kv = s_keys[p];
         const uint32_t d = CHEAP ? dig(kv) : s_dig[p];
         out[s_dst[d] + p] = kv;
+        if constexpr (KV) vout[s_dst[d] + p] = s_vals[p];
       }
     }
   }
