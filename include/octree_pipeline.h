@@ -272,6 +272,64 @@ int ocp_get_results(ocp_ctx ctx, ocp_results* out);
 /* Synchronises and copies the per-segment U and K (host arrays of n_seg). */
 int ocp_get_counts(ocp_ctx ctx, int32_t* n_unique, int32_t* n_oct);
 
+/* ------------------------------------------------------------ octree checks
+ * Structural check of the last build's node array on the device, in the role
+ * of checkTree (apps/cpu_demo/main.cpp:13-35, which the reference calls on
+ * its CPU output). The reference's offsets are an inclusive scan used as
+ * first slots (kernels/06_partial_sum.cpp:3-15, include/shared/oct.h:24-90),
+ * so chains overlap and child-in-octant does not hold in its own output; the
+ * counted violations are the properties that hold whatever the write order.
+ * The last two fields are information, not violations. Synchronises. */
+typedef struct ocp_tree_report {
+  int64_t slots;              /* K summed over segments */
+  int64_t written;            /* slots with a nonzero byte */
+  int64_t covered_unwritten;  /* violation: slot 0 or a chain slot left zero */
+  int64_t written_uncovered;  /* violation: written outside every chain */
+  int64_t chain_past_end;     /* violation: chain [P, P + e) not inside K */
+  int64_t bad_w;              /* violation: corner w != 1 */
+  int64_t bad_cell;           /* violation: cell size not range / 2^k */
+  int64_t off_grid;           /* violation: corner off its level's grid */
+  int64_t leaf_mask;          /* violation: child_leaf_mask != 0 */
+  int64_t bad_root;           /* violation: root corner or cell size */
+  int64_t links;              /* node-mask links */
+  int64_t links_in_octant;    /* of those, child inside the parent's octant */
+} ocp_tree_report;
+int ocp_validate_octree(ocp_ctx ctx, ocp_tree_report* report);
+
+/* Compact node format (12 B header per slot plus the nonzero children in one
+ * array), an exact encoding of the 60-byte OctNode of include/shared/
+ * types.h:28-34: the corner is the decode of a Morton prefix
+ * (include/shared/morton.h:45-80) and the cell size range / 2^depth
+ * (include/shared/oct.h:56-57, 87-88), so both are stored as integers. The
+ * reference has no compact form; this is its node array at a fifth of the
+ * bytes or less, for storage and transfer. */
+typedef struct ocp_compact_node {
+  uint32_t prefix;       /* Morton code of the corner (level's prefix) */
+  uint8_t level;         /* octree level | depth below the root << 4;
+                            0xff: slot never written (all zero) */
+  uint8_t node_mask;     /* child_node_mask */
+  uint8_t child_set;     /* bit c: children[c] != 0 */
+  uint8_t leaf_mask;     /* child_leaf_mask */
+  uint32_t first_child;  /* index of its first stored child */
+} ocp_compact_node;
+/* Compacts the last build's nodes (all segments, in oct_start order) into
+ * device arrays. With nodes or children NULL, only writes the sizes
+ * (n_nodes = K total, n_children). OCP_ERR_CAPACITY when a cap is below
+ * them; OCP_ERR_STATE when a slot has no exact compact form (not the case
+ * for the pipeline's own output). Synchronises. */
+int ocp_compact_octree(ocp_ctx ctx, ocp_compact_node* nodes, int64_t node_cap,
+                       int32_t* children, int64_t child_cap, int64_t* n_nodes,
+                       int64_t* n_children);
+/* 60-byte nodes from the compact form: oct_start (device, n_seg + 1) lays
+ * the segments out as in ocp_results; boxes are per segment (seg_min,
+ * seg_range device arrays) or, both NULL, min_coord and range. Child indices
+ * past n_children read as 0. Asynchronous. */
+int ocp_expand_octree(ocp_ctx ctx, const ocp_compact_node* nodes,
+                      const int32_t* children, int64_t n_children,
+                      const int32_t* oct_start, int32_t n_seg,
+                      const float* seg_min, const float* seg_range,
+                      float min_coord, float range, ocp_oct_node* out);
+
 /* Per-stage device time of the last eager build, in ms, measured with HIP
  * events on the context stream when stage timing is on: [morton, sort,
  * unique, radix_tree, edge_scan, octree, total]. */

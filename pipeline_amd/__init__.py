@@ -77,6 +77,26 @@ class LearnedStats(ctypes.Structure):
         return {f: int(getattr(self, f)) for f, _ in self._fields_ if f != "reserved"}
 
 
+class TreeReport(ctypes.Structure):
+    """ocp_tree_report: counted violations of the octree's structure, plus
+    the node-mask links and those whose child lies in the parent's octant."""
+    _fields_ = [(f, _I64) for f in (
+        "slots", "written", "covered_unwritten", "written_uncovered", "chain_past_end",
+        "bad_w", "bad_cell", "off_grid", "leaf_mask", "bad_root", "links",
+        "links_in_octant")]
+    VIOLATIONS = ("covered_unwritten", "written_uncovered", "chain_past_end", "bad_w",
+                  "bad_cell", "off_grid", "leaf_mask", "bad_root")
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
+# ocp_compact_node (12 bytes).
+COMPACT_NODE_DTYPE = np.dtype([("prefix", np.uint32), ("level", np.uint8),
+                               ("node_mask", np.uint8), ("child_set", np.uint8),
+                               ("leaf_mask", np.uint8), ("first_child", np.uint32)])
+assert COMPACT_NODE_DTYPE.itemsize == 12
+
 STATIC_SORT_MAX_KEYS = 1 << 24
 CDF_PARAMS = 12338
 CDF_ROOT_PARAMS = 49
@@ -122,6 +142,10 @@ SIGNATURES = {
     "ocp_set_graph_mode": (_INT, [_P, _INT]),
     "ocp_get_results": (_INT, [_P, ctypes.POINTER(Results)]),
     "ocp_get_counts": (_INT, [_P, _P, _P]),
+    "ocp_validate_octree": (_INT, [_P, ctypes.POINTER(TreeReport)]),
+    "ocp_compact_octree": (_INT, [_P, _P, _I64, _P, _I64, ctypes.POINTER(_I64),
+                                  ctypes.POINTER(_I64)]),
+    "ocp_expand_octree": (_INT, [_P, _P, _P, _I64, _P, _I32, _P, _P, _F, _F, _P]),
     "ocp_set_stage_timing": (_INT, [_P, _INT]),
     "ocp_get_stage_ms": (_INT, [_P, _P]),
     "ocp_get_kernel_ms": (_INT, [_P, _P, _P, _I32, ctypes.POINTER(_I32)]),
@@ -460,6 +484,38 @@ class Context:
         k = np.zeros(n_seg, np.int32)
         self._check(self.lib.ocp_get_counts(self.h, _ptr(u), _ptr(k)))
         return u, k
+
+    def validate_octree(self) -> dict:
+        """Structural check of the last build's nodes on the device (counts
+        per kind of violation; see TreeReport.VIOLATIONS)."""
+        rep = TreeReport()
+        self._check(self.lib.ocp_validate_octree(self.h, ctypes.byref(rep)))
+        return rep.as_dict()
+
+    def compact_octree(self):
+        """(nodes, children) device arrays of the last build's compact form."""
+        kn, kc = _I64(), _I64()
+        self._check(self.lib.ocp_compact_octree(self.h, None, 0, None, 0, ctypes.byref(kn),
+                                                ctypes.byref(kc)))
+        nodes = self.empty(max(kn.value, 1), COMPACT_NODE_DTYPE)
+        children = self.empty(max(kc.value, 1), np.int32)
+        self._check(self.lib.ocp_compact_octree(self.h, nodes.ptr, kn.value, children.ptr,
+                                                kc.value, ctypes.byref(kn), ctypes.byref(kc)))
+        nodes.shape, children.shape = (kn.value,), (kc.value,)
+        return nodes, children
+
+    def expand_octree(self, nodes: DeviceArray, children: DeviceArray, oct_start: int,
+                      n_seg: int, seg_min: int | None = None, seg_range: int | None = None,
+                      min_coord: float = 0.0, range_: float = 1024.0) -> DeviceArray:
+        """60-byte nodes from a compact form; oct_start, seg_min and seg_range
+        are device pointers (ocp_results fields). Returns a device array."""
+        out = self.empty(max(nodes.size, 1) * 60, np.uint8)
+        self._check(self.lib.ocp_expand_octree(self.h, nodes.ptr, children.ptr, children.size,
+                                               ctypes.c_void_p(oct_start), int(n_seg),
+                                               ctypes.c_void_p(seg_min) if seg_min else None,
+                                               ctypes.c_void_p(seg_range) if seg_range else None,
+                                               float(min_coord), float(range_), out.ptr))
+        return out
 
     def fetch(self, seg_offsets=None) -> BuildOutput:
         """Copy every stage output of the last build to the host."""

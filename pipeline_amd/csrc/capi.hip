@@ -1230,6 +1230,133 @@ int ocp_get_counts(ocp_ctx c, int32_t* n_unique, int32_t* n_oct) {
   return OCP_OK;
 }
 
+extern "C++" {
+namespace {
+
+// Device memory of one format call, freed when the call returns (after the
+// stream is synchronised).
+struct TempBuffers {
+  std::vector<void*> ptrs;
+  ~TempBuffers() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <typename T>
+  int alloc(ocp_ctx c, T** p, int64_t n) {
+    if (int r = dev_alloc(c, p, n)) return r;
+    ptrs.push_back(*p);
+    return OCP_OK;
+  }
+};
+
+// Total slots of the last build and each segment's root level (validator
+// and compact form). Settles the build first.
+int format_setup(ocp_ctx c, TempBuffers& tb, int64_t* k_total, int32_t** root_level,
+                 uint32_t** root_code) {
+  if (int r = settle(c)) return r;
+  const Workspace& w = c->ws;
+  const Plan& p = c->last_plan;
+  int32_t kt = 0;
+  OCP_HIP(c, hipMemcpyAsync(&kt, w.oct_start + p.n_seg, sizeof(int32_t), hipMemcpyDeviceToHost,
+                            c->stream));
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  *k_total = kt;
+  if (int r = tb.alloc(c, root_level, p.n_seg)) return r;
+  if (int r = tb.alloc(c, root_code, p.n_seg)) return r;
+  launch_seg_roots(w.keys_b, w.prefix_n, w.counts, p.sv.start, p.n_seg, *root_level, *root_code,
+                   c->stream);
+  OCP_CHECK_LAUNCH(c);
+  return OCP_OK;
+}
+
+}  // namespace
+}  // extern "C++"
+
+int ocp_validate_octree(ocp_ctx c, ocp_tree_report* out) {
+  if (check_ctx(c) || !out) return OCP_ERR_INVALID;
+  TempBuffers tb;
+  int64_t kt = 0;
+  int32_t* root_level = nullptr;
+  uint32_t* root_code = nullptr;
+  if (int r = format_setup(c, tb, &kt, &root_level, &root_code)) return r;
+  const Workspace& w = c->ws;
+  const Plan& p = c->last_plan;
+  uint8_t* cov = nullptr;
+  unsigned long long* rep = nullptr;
+  if (int r = tb.alloc(c, &cov, kt)) return r;
+  if (int r = tb.alloc(c, &rep, kTreeReportFields)) return r;
+  OCP_HIP(c, hipMemsetAsync(cov, 0, static_cast<size_t>(std::max<int64_t>(kt, 1)), c->stream));
+  OCP_HIP(c, hipMemsetAsync(rep, 0, sizeof(unsigned long long) * kTreeReportFields, c->stream));
+  launch_validate(w.nodes, w.edge, w.offs, w.counts, w.oct_start, p.sv.start, p.n_seg, p.sv.mn,
+                  p.sv.rg, p.sv.min1, p.sv.range1, root_level, root_code, cov, rep, c->stream);
+  OCP_CHECK_LAUNCH(c);
+  unsigned long long h[kTreeReportFields];
+  OCP_HIP(c, hipMemcpyAsync(h, rep, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  static_assert(sizeof(ocp_tree_report) == kTreeReportFields * sizeof(int64_t),
+                "report fields");
+  int64_t* o = reinterpret_cast<int64_t*>(out);
+  for (int f = 0; f < kTreeReportFields; ++f) o[f] = static_cast<int64_t>(h[f]);
+  return OCP_OK;
+}
+
+int ocp_compact_octree(ocp_ctx c, ocp_compact_node* nodes, int64_t node_cap, int32_t* children,
+                       int64_t child_cap, int64_t* n_nodes, int64_t* n_children) {
+  if (check_ctx(c) || !n_nodes || !n_children || node_cap < 0 || child_cap < 0)
+    return OCP_ERR_INVALID;
+  TempBuffers tb;
+  int64_t kt = 0;
+  int32_t* root_level = nullptr;
+  uint32_t* root_code = nullptr;
+  if (int r = format_setup(c, tb, &kt, &root_level, &root_code)) return r;
+  const Workspace& w = c->ws;
+  const Plan& p = c->last_plan;
+  int32_t* cnt = nullptr;
+  uint32_t* bsum = nullptr;
+  unsigned long long* bad = nullptr;
+  const int64_t nb = compact_scan_blocks(kt);
+  if (int r = tb.alloc(c, &cnt, kt)) return r;
+  if (int r = tb.alloc(c, &bsum, nb + 1)) return r;
+  if (int r = tb.alloc(c, &bad, 1)) return r;
+  OCP_HIP(c, hipMemsetAsync(bad, 0, sizeof(unsigned long long), c->stream));
+  launch_compact_count(w.nodes, w.oct_start, p.n_seg, p.sv.mn, p.sv.rg, p.sv.min1, p.sv.range1,
+                       root_level, kt, cnt, bsum, bad, c->stream);
+  OCP_CHECK_LAUNCH(c);
+  uint32_t total = 0;
+  unsigned long long hbad = 0;
+  OCP_HIP(c, hipMemcpyAsync(&total, bsum + nb, sizeof(total), hipMemcpyDeviceToHost, c->stream));
+  OCP_HIP(c, hipMemcpyAsync(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost, c->stream));
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  *n_nodes = kt;
+  *n_children = total;
+  if (hbad)
+    return fail(c, OCP_ERR_STATE, std::to_string(hbad) +
+                                      " octree slots have no exact compact form (corner off "
+                                      "its level's grid or cell size not range / 2^k)");
+  if (!nodes || !children) return OCP_OK;  // size query
+  if (node_cap < kt || child_cap < static_cast<int64_t>(total))
+    return fail(c, OCP_ERR_CAPACITY, "compact octree: output arrays too small (need " +
+                                         std::to_string(kt) + " nodes, " +
+                                         std::to_string(total) + " children)");
+  launch_compact_write(w.nodes, w.oct_start, p.n_seg, p.sv.mn, p.sv.rg, p.sv.min1, p.sv.range1,
+                       root_level, cnt, nodes, children, c->stream);
+  OCP_CHECK_LAUNCH(c);
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  return OCP_OK;
+}
+
+int ocp_expand_octree(ocp_ctx c, const ocp_compact_node* nodes, const int32_t* children,
+                      int64_t n_children, const int32_t* oct_start, int32_t n_seg,
+                      const float* seg_min, const float* seg_range, float min_coord, float range,
+                      ocp_oct_node* out) {
+  if (check_ctx(c) || !nodes || !oct_start || !out || n_seg < 1 || n_children < 0 ||
+      (n_children > 0 && !children) || (!seg_min) != (!seg_range))
+    return fail(c, OCP_ERR_INVALID, "expand octree: bad arguments");
+  launch_expand(nodes, children, n_children, oct_start, n_seg, seg_min, seg_range, min_coord,
+                range, reinterpret_cast<OctNode*>(out), c->stream);
+  OCP_CHECK_LAUNCH(c);
+  return OCP_OK;
+}
+
 int ocp_set_stage_timing(ocp_ctx c, int enable) {
   if (check_ctx(c)) return OCP_ERR_INVALID;
   c->stage_timing = enable != 0;

@@ -213,4 +213,33 @@ void launch_oct_finalize(const OctArgs& a, const SegView& sv,
                          const int32_t* tile_start, int32_t total_tiles,
                          int resolve, hipStream_t s);
 
+// ---- node formats and validator (k_format.hip). Node counts come from
+// oct_start[n_seg] on the device; boxes are per segment (smin/srg) or the
+// scalars mn1/rg1.
+constexpr int kTreeReportFields = 12;
+void launch_seg_roots(const uint32_t* codes, const uint8_t* prefix_n, const int32_t* n_unique,
+                      const int32_t* seg_start, int32_t n_seg, int32_t* root_level,
+                      uint32_t* root_code, hipStream_t s);
+// cov: one byte per slot, zeroed; rep: kTreeReportFields counters, zeroed.
+void launch_validate(const OctNode* nodes, const int32_t* edge, const int32_t* offs,
+                     const int32_t* n_unique, const int32_t* oct_start, const int32_t* seg_start,
+                     int32_t n_seg, const float* smin, const float* srg, float mn1, float rg1,
+                     const int32_t* root_level, const uint32_t* root_code, uint8_t* cov,
+                     unsigned long long* rep, hipStream_t s);
+// Scan blocks of the compact pass (bsum holds this + 1 words).
+int64_t compact_scan_blocks(int64_t k_total);
+// cnt[k_total] <- inclusive scan of nonzero children per slot; bsum[nb] =
+// total; *bad += slots without an exact compact form (zeroed beforehand).
+void launch_compact_count(const OctNode* nodes, const int32_t* oct_start, int32_t n_seg,
+                          const float* smin, const float* srg, float mn1, float rg1,
+                          const int32_t* root_level, int64_t k_total, int32_t* cnt,
+                          uint32_t* bsum, unsigned long long* bad, hipStream_t s);
+void launch_compact_write(const OctNode* nodes, const int32_t* oct_start, int32_t n_seg,
+                          const float* smin, const float* srg, float mn1, float rg1,
+                          const int32_t* root_level, const int32_t* incl, void* out,
+                          int32_t* children, hipStream_t s);
+void launch_expand(const void* in, const int32_t* children, int64_t n_children,
+                   const int32_t* oct_start, int32_t n_seg, const float* smin, const float* srg,
+                   float mn1, float rg1, OctNode* out, hipStream_t s);
+
 }  // namespace ocp

@@ -60,10 +60,26 @@ def check_octree(nodes, codes, prefix_n, edge, offsets, min_coord=0.0, range_=10
     root_level = int(prefix_n[0]) // 3 if m >= 1 else 0
     level = root_level + np.round(lv).astype(np.int64)
     assert level.max(initial=0) <= 11, "level beyond the code length"
-    q = (corner[w, :3] - min_coord) / range_ * 1024.0
-    assert np.all(q == np.round(q)) and q.min() >= 0 and q.max() < 1024, "corner off the grid"
-    step = 2.0 ** np.maximum(10 - level, 0)
-    assert np.all(np.mod(q, step[:, None]) == 0), "corner not on its level's grid"
+    # The corner is the float32 decode q / 1024 * range + min of an axis value
+    # q on the level's grid: some q near the real-valued inverse decodes to
+    # it bit for bit (for a box other than [0, 1024) the inverse is not an
+    # integer itself).
+    q = np.floor((corner[w, :3] - min_coord) / range_ * 1024.0)
+    step = (2 ** np.maximum(10 - level, 0))[:, None]
+    f32 = np.float32
+    got = nodes["corner"][w, :3]
+    ok = np.zeros(got.shape, bool)
+    on_any = np.zeros(got.shape, bool)
+    for d in (-1, 0, 1, 2):
+        cand = q + d
+        inside = (cand >= 0) & (cand < 1024)
+        dec = (np.clip(cand, 0, 1023).astype(f32) / f32(1024.0) * f32(range_)).astype(f32) \
+            + f32(min_coord)
+        hit = inside & (dec == got)
+        on_any |= hit
+        ok |= hit & (np.mod(cand, step) == 0)
+    assert np.all(on_any), "corner off the grid"
+    assert np.all(ok), "corner not on its level's grid"
     # root: corner of codes[0]'s root-level prefix, cell size = range
     sh = (30 - 3 * root_level) & 31
     c0 = (int(codes[0]) >> sh) << sh

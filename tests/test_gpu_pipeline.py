@@ -7,7 +7,7 @@ import pytest
 
 import oracle
 from octree_checks import check_octree
-from pipeline_amd import clouds
+from pipeline_amd import TreeReport, clouds
 
 pytestmark = pytest.mark.gpu
 
@@ -151,6 +151,21 @@ def test_batched_512_clouds_full_workload(ctx):
         d.free()
 
 
+def _device_checks(ctx, out, written):
+    """The device validator agrees with the CPU structure check, and the
+    compact form expands back to the same node bytes."""
+    rep = ctx.validate_octree()
+    assert all(rep[f] == 0 for f in TreeReport.VIOLATIONS), rep
+    assert rep["written"] == written
+    cn, ch = ctx.compact_octree()
+    r = ctx.results()
+    k = int(out.n_oct[0])
+    back = ctx.expand_octree(cn, ch, r.oct_start, r.n_seg, None, None, 0.0, 1024.0)
+    np.testing.assert_array_equal(back.to_host(k * 60), out.nodes[0].view(np.uint8))
+    for a in (cn, ch, back):
+        a.free()
+
+
 def test_plain_10M_uniform_full_oracle(ctx):
     """The plain GPU workload at its real size: 10M uniform float4 points,
     seed 114514, every stage output against the oracle, plus the octree's
@@ -159,9 +174,10 @@ def test_plain_10M_uniform_full_oracle(ctx):
     pts = clouds.uniform(10_000_000, seed=114514)
     exp = oracle.build_plain(pts)
     out = _plain(ctx, pts)
-    check_octree(out.nodes[0], out.unique_keys[0], out.prefix_n[0], out.edge_count[0],
-                 out.offsets[0])
+    written = check_octree(out.nodes[0], out.unique_keys[0], out.prefix_n[0],
+                           out.edge_count[0], out.offsets[0])
     _check_segment(out, 0, exp)
+    _device_checks(ctx, out, written)
 
 
 def test_plain_2p24_clustered_full_oracle(ctx):
@@ -171,9 +187,10 @@ def test_plain_2p24_clustered_full_oracle(ctx):
     exp = oracle.build_plain(pts)
     assert exp["n_unique"] < (1 << 24) // 2  # the duplicate-heavy case
     out = _plain(ctx, pts)
-    check_octree(out.nodes[0], out.unique_keys[0], out.prefix_n[0], out.edge_count[0],
-                 out.offsets[0])
+    written = check_octree(out.nodes[0], out.unique_keys[0], out.prefix_n[0],
+                           out.edge_count[0], out.offsets[0])
     _check_segment(out, 0, exp)
+    _device_checks(ctx, out, written)
 
 
 def test_tuned_configs_give_identical_outputs(ctx):
