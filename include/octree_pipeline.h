@@ -272,6 +272,36 @@ int ocp_get_results(ocp_ctx ctx, ocp_results* out);
 /* Synchronises and copies the per-segment U and K (host arrays of n_seg). */
 int ocp_get_counts(ocp_ctx ctx, int32_t* n_unique, int32_t* n_oct);
 
+/* ------------------------------------------------------------ streaming
+ * The pipelined demo `ppl` that apps/xmake.lua:7 and apps/README.md:5 name
+ * (its sources are not in the reference): a stream of frames, each built
+ * as by ocp_build, with consecutive frames overlapping on the device.
+ * Frames rotate over `depth` slots; each slot is a context with its own
+ * stream and outputs, a pinned host buffer and a device points buffer of
+ * max_points float4. A slot's upload and build run in order on its stream,
+ * so one frame's upload and Morton/radix-tree/octree kernels run beside
+ * another frame's work; the sort, unique and scan launches of two frames
+ * are ordered (their tiles wait on each other inside a launch). Slot
+ * contexts run without graphs. */
+typedef struct ocp_pipeline* ocp_ppl;
+int ocp_ppl_create(ocp_ppl* out, int device, int32_t depth, int64_t max_points);
+int ocp_ppl_destroy(ocp_ppl ppl);
+const char* ocp_ppl_last_error(ocp_ppl ppl);
+/* The slot's pinned host buffer (max_points float4, or float3 packed), for
+ * the producer to write the next frame into; waits until the slot's last
+ * upload from it has finished. */
+int ocp_ppl_host_buffer(ocp_ppl ppl, int32_t slot, float** host);
+/* Enqueues one frame on a slot: points is a device array, or NULL to upload
+ * n points from the slot's host buffer first. The slot's previous outputs
+ * are overwritten: read them (ocp_ppl_wait, then ocp_get_results on
+ * ocp_ppl_context) before submitting to the slot again. Asynchronous. */
+int ocp_ppl_submit(ocp_ppl ppl, int32_t slot, const float* points, int float3,
+                   int64_t n, float min_coord, float range);
+/* Waits for the slot's frame; reports its errors as ocp_synchronize does. */
+int ocp_ppl_wait(ocp_ppl ppl, int32_t slot);
+/* The slot's context: its results, validator and compact form. */
+int ocp_ppl_context(ocp_ppl ppl, int32_t slot, ocp_ctx* ctx);
+
 /* ------------------------------------------------------------ octree checks
  * Structural check of the last build's node array on the device, in the role
  * of checkTree (apps/cpu_demo/main.cpp:13-35, which the reference calls on
@@ -334,6 +364,12 @@ int ocp_expand_octree(ocp_ctx ctx, const ocp_compact_node* nodes,
  * events on the context stream when stage timing is on: [morton, sort,
  * unique, radix_tree, edge_scan, octree, total]. */
 int ocp_set_stage_timing(ocp_ctx ctx, int enable);
+/* Cold per-kernel timing: with flush_bytes > 0 (above the 256 MB Infinity
+ * Cache, e.g. 512 MiB), each kernel timed by stage timing is preceded by a
+ * fill of a context-owned buffer of that size, outside its events - the
+ * role of CudaEventTimer's L2 flush (benchmarks/cuda/
+ * cuda_bench_helper.cuh:69-90). 0 frees the buffer. Measurement only. */
+int ocp_set_cold_timing(ocp_ctx ctx, int64_t flush_bytes);
 int ocp_get_stage_ms(ocp_ctx ctx, float* ms7);
 /* Per-kernel device time of the last eager build with stage timing on, in
  * launch order, each bracketed by HIP events on the context stream. Writes

@@ -142,11 +142,19 @@ SIGNATURES = {
     "ocp_set_graph_mode": (_INT, [_P, _INT]),
     "ocp_get_results": (_INT, [_P, ctypes.POINTER(Results)]),
     "ocp_get_counts": (_INT, [_P, _P, _P]),
+    "ocp_ppl_create": (_INT, [ctypes.POINTER(_P), _INT, _I32, _I64]),
+    "ocp_ppl_destroy": (_INT, [_P]),
+    "ocp_ppl_last_error": (ctypes.c_char_p, [_P]),
+    "ocp_ppl_host_buffer": (_INT, [_P, _I32, ctypes.POINTER(_P)]),
+    "ocp_ppl_submit": (_INT, [_P, _I32, _P, _INT, _I64, _F, _F]),
+    "ocp_ppl_wait": (_INT, [_P, _I32]),
+    "ocp_ppl_context": (_INT, [_P, _I32, ctypes.POINTER(_P)]),
     "ocp_validate_octree": (_INT, [_P, ctypes.POINTER(TreeReport)]),
     "ocp_compact_octree": (_INT, [_P, _P, _I64, _P, _I64, ctypes.POINTER(_I64),
                                   ctypes.POINTER(_I64)]),
     "ocp_expand_octree": (_INT, [_P, _P, _P, _I64, _P, _I32, _P, _P, _F, _F, _P]),
     "ocp_set_stage_timing": (_INT, [_P, _INT]),
+    "ocp_set_cold_timing": (_INT, [_P, _I64]),
     "ocp_get_stage_ms": (_INT, [_P, _P]),
     "ocp_get_kernel_ms": (_INT, [_P, _P, _P, _I32, ctypes.POINTER(_I32)]),
     "ocp_time_builds": (_INT, [_P, _P, _INT, _P, _P, _P, _I32, _INT, _INT, ctypes.POINTER(_F)]),
@@ -427,6 +435,10 @@ class Context:
     def set_stage_timing(self, enable: bool):
         self._check(self.lib.ocp_set_stage_timing(self.h, int(bool(enable))))
 
+    def set_cold_timing(self, flush_bytes: int):
+        """Overwrite a buffer of flush_bytes before every timed kernel (0: off)."""
+        self._check(self.lib.ocp_set_cold_timing(self.h, int(flush_bytes)))
+
     def stage_ms(self) -> dict:
         ms = np.zeros(7, np.float32)
         self._check(self.lib.ocp_get_stage_ms(self.h, _ptr(ms)))
@@ -690,3 +702,92 @@ class Context:
         self._check(self.lib.ocp_sort_learned(self.h, keys.ptr, keys_alt.ptr, int(n),
                                               ctypes.byref(st)))
         return st.as_dict()
+
+
+class _SlotContext(Context):
+    """A pipeline slot's context: the Context API over a handle the pipeline
+    owns (closing it frees only the arrays allocated through it)."""
+
+    def __init__(self, lib, handle):
+        self.lib = lib
+        self.h = ctypes.c_void_p(handle)
+        self._owned = []
+
+    def close(self):
+        for a in self._owned:
+            a.free()
+        self._owned.clear()
+        self.h = None
+
+
+class Pipeline:
+    """A stream of frames built as by Context.build, consecutive frames
+    overlapping on the device (ocp_ppl_*). Frames rotate over `depth` slots;
+    each slot has its own stream, outputs, pinned host buffer and device
+    points buffer of max_points float4."""
+
+    def __init__(self, depth: int = 2, max_points: int = 0, device: int = 0):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self.lib.ocp_ppl_create(ctypes.byref(h), device, int(depth), int(max_points))
+        if rc != OCP_OK:
+            raise OcpError(f"ocp_ppl_create failed with status {rc}")
+        self.h = h
+        self.depth = int(depth)
+        self.max_points = int(max_points)
+        self._slots = []
+        for i in range(self.depth):
+            c = ctypes.c_void_p()
+            self._check(self.lib.ocp_ppl_context(self.h, i, ctypes.byref(c)))
+            self._slots.append(_SlotContext(self.lib, c.value))
+
+    def _check(self, rc: int):
+        if rc != OCP_OK:
+            msg = self.lib.ocp_ppl_last_error(self.h)
+            raise OcpError(f"status {rc}: {msg.decode() if msg else ''}")
+
+    def host_buffer(self, slot: int, n: int | None = None, float3: bool = False) -> np.ndarray:
+        """The slot's pinned host buffer as an (n, 4) (or (n, 3)) float32 view,
+        once its last upload has finished."""
+        ptr = ctypes.c_void_p()
+        self._check(self.lib.ocp_ppl_host_buffer(self.h, int(slot), ctypes.byref(ptr)))
+        n = self.max_points if n is None else int(n)
+        w = 3 if float3 else 4
+        buf = (ctypes.c_float * (n * w)).from_address(ptr.value)
+        return np.ctypeslib.as_array(buf).reshape(n, w)
+
+    def submit(self, slot: int, points: DeviceArray | None, n: int, min_coord: float = 0.0,
+               range_: float = 1024.0, float3: bool = False):
+        """Enqueue one frame (points None: upload n points from the slot's
+        host buffer first)."""
+        if points is not None:
+            float3 = points.shape[-1] == 3
+        self._check(self.lib.ocp_ppl_submit(self.h, int(slot),
+                                            points.ptr if points is not None else None,
+                                            int(float3), int(n), float(min_coord),
+                                            float(range_)))
+
+    def wait(self, slot: int):
+        self._check(self.lib.ocp_ppl_wait(self.h, int(slot)))
+
+    def context(self, slot: int) -> Context:
+        return self._slots[slot]
+
+    def close(self):
+        if getattr(self, "h", None):
+            for c in self._slots:
+                c.close()
+            self.lib.ocp_ppl_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -180,6 +180,16 @@ struct ocp_context {
   Plan last_plan;
   bool graph_mode = false;
   bool permutation = false;
+  // Streaming: the first lookback launch of a build waits for order_wait,
+  // and order_record is recorded after its last one (eager builds only).
+  hipEvent_t order_wait = nullptr;
+  hipEvent_t order_record = nullptr;
+  hipError_t order_err = hipSuccess;
+  // Cold timing: with stage timing on, this buffer is overwritten before
+  // every timed kernel of an eager build (evicts L2 and the Infinity Cache).
+  uint32_t* cold_buf = nullptr;
+  int64_t cold_words = 0;
+  uint32_t cold_seq = 0;
   std::map<std::string, hipGraphExec_t> graphs;
   bool stage_timing = false;
   hipEvent_t ev[8] = {};
@@ -464,6 +474,8 @@ struct KScope {
   bool on;
   KScope(ocp_ctx c_, int id_, bool capturing)
       : c(c_), id(id_), on(c_->stage_timing && !capturing) {
+    if (on && c->cold_buf)
+      launch_fill_u32(c->cold_buf, ++c->cold_seq, c->cold_words, c->stream);
     if (on) (void)hipEventRecord(c->kev[2 * id], c->stream);
   }
   ~KScope() {
@@ -534,6 +546,10 @@ bool enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
   uint32_t* dst = w.keys_b;
   uint32_t* vsrc = nullptr;  // the first pass numbers the points itself
   uint32_t* vdst = w.vals_b;
+  // Lookback launches of two builds never run at the same time (each relies
+  // on its lowest unfinished tile being resident); the rest may overlap.
+  if (c->order_wait && !capturing && c->order_err == hipSuccess)
+    c->order_err = hipStreamWaitEvent(s, c->order_wait, 0);
   for (int pass = 0; pass < 4; ++pass) {
     KScope k(c, kKPass0 + pass, capturing);
     if (p.permutation) {
@@ -563,6 +579,8 @@ bool enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
   { KScope k(c, kKEdgeScan, capturing);
     launch_edge_scan(w.prefix_n, w.par, w.ppn, w.edge, w.offs, p.sv, p.ts_scan, p.tiles_scan,
                      n_unique, 0, w.flags, c->ctrl, kalloc, 0, tt, s); }
+  if (c->order_record && !capturing && c->order_err == hipSuccess)
+    c->order_err = hipEventRecord(c->order_record, s);
   mark(c, 5, capturing);
   const bool ok = enqueue_octree(c, p, capturing);
   mark(c, 6, capturing);
@@ -614,7 +632,9 @@ int run_plan(ocp_ctx c, const float* pts, int f3, const Plan& p,
     OCP_HIP(c, hipGraphLaunch(it->second, c->stream));
     c->ev_recorded = false;
   } else {
+    c->order_err = hipSuccess;
     const bool ok = enqueue_pipeline(c, pts, f3 != 0, p, false);
+    OCP_HIP(c, c->order_err);
     OCP_CHECK_LAUNCH(c);
     if (!ok) return fail(c, OCP_ERR_INVALID, "octree stage: a stage array is missing");
     c->ev_recorded = c->stage_timing;
@@ -740,6 +760,7 @@ int ocp_destroy(ocp_ctx c) {
   for (auto& e : c->kev)
     if (e) (void)hipEventDestroy(e);
   free_learn(c->learn);
+  dev_free(c->cold_buf);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return OCP_OK;
@@ -1357,6 +1378,19 @@ int ocp_expand_octree(ocp_ctx c, const ocp_compact_node* nodes, const int32_t* c
   return OCP_OK;
 }
 
+int ocp_set_cold_timing(ocp_ctx c, int64_t flush_bytes) {
+  if (check_ctx(c) || flush_bytes < 0) return OCP_ERR_INVALID;
+  OCP_HIP(c, hipStreamSynchronize(c->stream));
+  dev_free(c->cold_buf);
+  c->cold_words = 0;
+  if (flush_bytes > 0) {
+    const int64_t words = (flush_bytes + 3) / 4;
+    if (int r = dev_alloc(c, &c->cold_buf, words)) return r;
+    c->cold_words = words;
+  }
+  return OCP_OK;
+}
+
 int ocp_set_stage_timing(ocp_ctx c, int enable) {
   if (check_ctx(c)) return OCP_ERR_INVALID;
   c->stage_timing = enable != 0;
@@ -1814,6 +1848,141 @@ int ocp_sort_learned(ocp_ctx c, uint32_t* keys, uint32_t* alt, uint32_t n,
     local.full_resort = 1;
   }
   if (st) *st = local;
+  return OCP_OK;
+}
+
+// ---------------------------------------------------------------- streaming
+// Frames rotate over `depth` contexts, each with its own stream, workspace,
+// pinned host staging buffer and device points buffer. A frame's upload and
+// kernels run in order on its slot's stream, so slots overlap: one frame's
+// upload (a DMA engine) and Morton, radix-tree and octree kernels run beside
+// another's. Lookback launches (sort passes, unique, edge scan) of two frames
+// never overlap: each build's first one waits on the previous frame's last.
+struct ocp_pipeline {
+  int device = 0;
+  int64_t max_points = 0;
+  std::vector<ocp_ctx> slots;
+  std::vector<hipEvent_t> scan_done;    // after a slot's last lookback launch
+  std::vector<hipEvent_t> upload_done;  // after the upload from a slot's host buffer
+  std::vector<float*> host;             // pinned, max_points float4 per slot
+  std::vector<float*> dev;              // device, max_points float4 per slot
+  int32_t last_slot = -1;
+  std::string err;
+};
+
+static int ppl_fail(ocp_ppl p, int code, const std::string& msg) {
+  p->err = msg;
+  return code;
+}
+
+static int ppl_slot(ocp_ppl p, int32_t slot) {
+  if (!p) return OCP_ERR_INVALID;
+  if (slot < 0 || slot >= static_cast<int32_t>(p->slots.size()))
+    return ppl_fail(p, OCP_ERR_INVALID, "slot out of range");
+  return OCP_OK;
+}
+
+int ocp_ppl_destroy(ocp_ppl p) {
+  if (!p) return OCP_ERR_INVALID;
+  for (ocp_ctx c : p->slots)
+    if (c) (void)hipStreamSynchronize(c->stream);
+  for (float* h : p->host)
+    if (h) (void)hipHostFree(h);
+  for (float* d : p->dev)
+    if (d) (void)hipFree(d);
+  for (hipEvent_t e : p->scan_done)
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : p->upload_done)
+    if (e) (void)hipEventDestroy(e);
+  for (ocp_ctx c : p->slots)
+    if (c) (void)ocp_destroy(c);
+  delete p;
+  return OCP_OK;
+}
+
+int ocp_ppl_create(ocp_ppl* out, int device, int32_t depth, int64_t max_points) {
+  if (!out) return OCP_ERR_INVALID;
+  *out = nullptr;
+  if (depth < 1 || depth > 8 || max_points < 0 || max_points > kMaxSegPoints)
+    return OCP_ERR_INVALID;
+  auto* p = new ocp_pipeline();
+  p->device = device;
+  p->max_points = max_points;
+  p->slots.assign(depth, nullptr);
+  p->scan_done.assign(depth, nullptr);
+  p->upload_done.assign(depth, nullptr);
+  p->host.assign(depth, nullptr);
+  p->dev.assign(depth, nullptr);
+  const size_t bytes = sizeof(float) * 4 * static_cast<size_t>(max_points);
+  for (int32_t i = 0; i < depth; ++i) {
+    if (int r = ocp_create(&p->slots[i], device, nullptr)) {
+      ocp_ppl_destroy(p);
+      return r;
+    }
+    if (hipEventCreateWithFlags(&p->scan_done[i], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&p->upload_done[i], hipEventDisableTiming) != hipSuccess ||
+        (bytes && (hipHostMalloc(reinterpret_cast<void**>(&p->host[i]), bytes,
+                                 hipHostMallocDefault) != hipSuccess ||
+                   hipMalloc(reinterpret_cast<void**>(&p->dev[i]), bytes) != hipSuccess))) {
+      ocp_ppl_destroy(p);
+      return OCP_ERR_HIP;
+    }
+  }
+  *out = p;
+  return OCP_OK;
+}
+
+const char* ocp_ppl_last_error(ocp_ppl p) { return p ? p->err.c_str() : "null pipeline"; }
+
+int ocp_ppl_host_buffer(ocp_ppl p, int32_t slot, float** host) {
+  if (int r = ppl_slot(p, slot)) return r;
+  if (!host) return ppl_fail(p, OCP_ERR_INVALID, "null output");
+  if (!p->host[slot]) return ppl_fail(p, OCP_ERR_STATE, "created with max_points = 0");
+  if (hipEventSynchronize(p->upload_done[slot]) != hipSuccess)
+    return ppl_fail(p, OCP_ERR_HIP, "waiting for the slot's last upload failed");
+  *host = p->host[slot];
+  return OCP_OK;
+}
+
+int ocp_ppl_submit(ocp_ppl p, int32_t slot, const float* points, int float3, int64_t n,
+                   float min_coord, float range) {
+  if (int r = ppl_slot(p, slot)) return r;
+  ocp_ctx c = p->slots[slot];
+  const float* pts = points;
+  if (!points) {
+    if (n < 0 || n > p->max_points)
+      return ppl_fail(p, OCP_ERR_CAPACITY, "a staged frame holds at most max_points points");
+    const size_t bytes = static_cast<size_t>(n) * (float3 ? 12u : 16u);
+    if (bytes) {
+      hipError_t e = hipMemcpyAsync(p->dev[slot], p->host[slot], bytes, hipMemcpyHostToDevice,
+                                    c->stream);
+      if (e == hipSuccess) e = hipEventRecord(p->upload_done[slot], c->stream);
+      if (e != hipSuccess)
+        return ppl_fail(p, OCP_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
+    }
+    pts = p->dev[slot];
+  }
+  c->graph_mode = false;
+  c->order_wait = p->last_slot >= 0 ? p->scan_done[p->last_slot] : nullptr;
+  c->order_record = p->scan_done[slot];
+  const int r = ocp_build(c, pts, float3, n, min_coord, range);
+  c->order_wait = nullptr;
+  c->order_record = nullptr;
+  if (r) return ppl_fail(p, r, c->err);
+  p->last_slot = slot;
+  return OCP_OK;
+}
+
+int ocp_ppl_wait(ocp_ppl p, int32_t slot) {
+  if (int r = ppl_slot(p, slot)) return r;
+  if (int r = ocp_synchronize(p->slots[slot])) return ppl_fail(p, r, p->slots[slot]->err);
+  return OCP_OK;
+}
+
+int ocp_ppl_context(ocp_ppl p, int32_t slot, ocp_ctx* ctx) {
+  if (int r = ppl_slot(p, slot)) return r;
+  if (!ctx) return ppl_fail(p, OCP_ERR_INVALID, "null output");
+  *ctx = p->slots[slot];
   return OCP_OK;
 }
 
