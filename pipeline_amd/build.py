@@ -53,6 +53,7 @@ def build(verbose: bool = True) -> str:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
     build_c_demo()
+    build_fixture_demo()
     if verbose:
         print(f"built {LIB}")
     return LIB
@@ -75,6 +76,24 @@ def build_c_demo(out: str = C_DEMO) -> str:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"C demo build failed:\n{r.stdout}\n{r.stderr}")
+    return out
+
+
+FIXTURE_DEMO_SRC = os.path.join(ROOT, "tests", "c", "fixture_demo.cpp")
+FIXTURE_DEMO = os.path.join(OUT_DIR, "fixture_demo")
+
+
+def build_fixture_demo(out: str = FIXTURE_DEMO) -> str:
+    """Compiles the C++ caller of include/octree_fixture.hpp with g++ (host
+    code only) and links it against the library."""
+    cxx = os.environ.get("CXX", shutil.which("g++") or "c++")
+    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-I",
+           os.path.join(ROOT, "include"), FIXTURE_DEMO_SRC, "-o", out, "-L", OUT_DIR,
+           "-locpipe", "-L", rocm_lib, f"-Wl,-rpath,{OUT_DIR}", f"-Wl,-rpath,{rocm_lib}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"fixture demo build failed:\n{r.stdout}\n{r.stderr}")
     return out
 
 

@@ -105,3 +105,30 @@ def test_c_caller_runs():
     r = subprocess.run([B.C_DEMO, str(exp["n_unique"]), str(exp["n_oct"])],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_fixture_caller_compiles_and_links(tmp_path):
+    """include/octree_fixture.hpp compiles with g++ -Wall -Wextra -Werror and
+    its caller links against the library."""
+    from pipeline_amd import build as B
+    exe = B.build_fixture_demo(str(tmp_path / "fixture_demo"))
+    assert os.path.exists(exe)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [307_200, 10_000_000])
+def test_fixture_matches_fused_build_and_oracle(tmp_path, n):
+    """ocp::GpuFixture (benchmarks/cuda/fixture.cuh:24-91 over the C ABI) plus
+    the per-stage octree entries give the fused build's bytes (checked inside
+    the program), and its counts are the oracle's on the same points."""
+    from pipeline_amd import build as B
+    assert os.path.exists(B.FIXTURE_DEMO), "build the library first (pipeline_amd/build.py)"
+    path = tmp_path / "points.f32"
+    r = subprocess.run([B.FIXTURE_DEMO, str(n), str(path)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    pts = np.fromfile(path, np.float32).reshape(n, 4)
+    assert np.all(pts[:, 3] == 1.0) and pts[:, :3].min() >= 0 and pts[:, :3].max() < 1024
+    exp = oracle.build_plain(pts)
+    u, k = (int(x) for x in r.stdout.split())
+    assert (u, k) == (exp["n_unique"], exp["n_oct"])
