@@ -376,6 +376,8 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
+    ap.add_argument("--tune-cache", default=None,
+                    help="tuning cache file: reuse a tuned workload class instead of timing")
     ap.add_argument("--learned-frames", type=int, default=4,
                     help="frames of the learned-sort comparison (30 for the full sequence)")
     args = ap.parse_args()
@@ -408,7 +410,7 @@ def main() -> int:
         b_buf.free()
     tuned = None
     if not args.no_autotune:
-        tuned = ctx.autotune(d, n=n, reps=3)
+        tuned = ctx.autotune(d, n=n, reps=3, cache=args.tune_cache)
     # Eager builds with per-kernel HIP events: back to back (warm), then with
     # 512 MiB written before every kernel (cold: L2 and the 256 MB Infinity
     # Cache hold none of its input).
@@ -481,7 +483,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": wl, "n_points": n, "n_unique": u, "n_oct": k,
                    "seed": 114514, "hipgraph": True, "tune": ctx.get_tune(),
-                   "autotuned": tuned is not None},
+                   "autotuned": tuned is not None,
+                   "tune_from_cache": bool(getattr(ctx, "tune_from_cache", False))},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -386,13 +386,17 @@ int ocp_time_builds(ocp_ctx ctx, const float* points, int float3,
                     int steps, float* ms_per_build);
 
 /* ------------------------------------------------------------ tuning */
-/* Tile parameters: sort_waves, sort_kpt, morton_ppt, morton_grid, scan_ipt. */
+/* Tile parameters and the sort's digit width (pass count). */
 typedef struct ocp_tune {
-  int32_t sort_waves;
-  int32_t sort_kpt;
+  int32_t sort_waves;   /* OneSweep tile of 8-bit digits: waves ... */
+  int32_t sort_kpt;     /* ... and keys per thread */
   int32_t morton_ppt;
   int32_t morton_grid;
   int32_t scan_ipt;
+  int32_t sort_bits;    /* digit width of a build's sort: 8 (four passes) or
+                           10 (three passes over the 30-bit keys; builds with
+                           the permutation keep 8) */
+  int32_t sort10_kpt;   /* keys per thread of a 10-bit pass (8 or 16) */
 } ocp_tune;
 int ocp_get_tune(ocp_ctx ctx, ocp_tune* out);
 int ocp_set_tune(ocp_ctx ctx, const ocp_tune* in);
@@ -402,6 +406,16 @@ int ocp_autotune(ocp_ctx ctx, const float* points, int float3, int64_t n,
                  const int32_t* seg_offsets, const float* seg_min,
                  const float* seg_range, int32_t n_seg, int reps,
                  ocp_tune* chosen);
+/* ocp_autotune with its result kept in a text file, one line per workload
+ * class (device architecture and compute units, plain or batched, point
+ * layout, point and segment counts rounded up to powers of two): a class
+ * already in the file is applied without timing anything (*from_cache = 1),
+ * otherwise it is tuned and its line appended. */
+int ocp_autotune_cached(ocp_ctx ctx, const char* cache_path, const float* points,
+                        int float3, int64_t n, const int32_t* seg_offsets,
+                        const float* seg_min, const float* seg_range,
+                        int32_t n_seg, int reps, ocp_tune* chosen,
+                        int* from_cache);
 
 /* ------------------------------------------------------------ learned sort
  * No counterpart in the reference, which sorts with OneSweep only

@@ -62,7 +62,8 @@ class Results(ctypes.Structure):
 
 class Tune(ctypes.Structure):
     _fields_ = [("sort_waves", _I32), ("sort_kpt", _I32), ("morton_ppt", _I32),
-                ("morton_grid", _I32), ("scan_ipt", _I32)]
+                ("morton_grid", _I32), ("scan_ipt", _I32), ("sort_bits", _I32),
+                ("sort10_kpt", _I32)]
 
     def as_dict(self):
         return {f: int(getattr(self, f)) for f, _ in self._fields_}
@@ -161,6 +162,8 @@ SIGNATURES = {
     "ocp_get_tune": (_INT, [_P, ctypes.POINTER(Tune)]),
     "ocp_set_tune": (_INT, [_P, ctypes.POINTER(Tune)]),
     "ocp_autotune": (_INT, [_P, _P, _INT, _I64, _P, _P, _P, _I32, _INT, ctypes.POINTER(Tune)]),
+    "ocp_autotune_cached": (_INT, [_P, ctypes.c_char_p, _P, _INT, _I64, _P, _P, _P, _I32, _INT,
+                                   ctypes.POINTER(Tune), ctypes.POINTER(_INT)]),
     "ocp_cdf_reset": (_INT, [_P, _U32, _U32]),
     "ocp_cdf_add_samples": (_INT, [_P, _P, _I64, _I32]),
     "ocp_cdf_clear_samples": (_INT, [_P]),
@@ -609,22 +612,32 @@ class Context:
         self._check(self.lib.ocp_set_tune(self.h, ctypes.byref(t)))
 
     def autotune(self, points: DeviceArray, n=None, seg_offsets=None, seg_min=None,
-                 seg_range=None, reps=3) -> dict:
+                 seg_range=None, reps=3, cache: str | None = None) -> dict:
+        """Time the tile configurations and digit widths on this workload and
+        keep the fastest. With `cache` (a text file), a workload class already
+        tuned there is applied without timing (self.tune_from_cache)."""
         t = Tune()
         f3 = int(points.shape[-1] == 3)
         if seg_offsets is None:
             n = points.shape[0] if n is None else n
+            offs = None
             mn = np.array([0.0], np.float32)
             rg = np.array([1024.0], np.float32)
-            self._check(self.lib.ocp_autotune(self.h, points.ptr, f3, n, None, _ptr(mn),
-                                              _ptr(rg), 0, reps, ctypes.byref(t)))
+            args = (points.ptr, f3, int(n), None, _ptr(mn), _ptr(rg), 0, reps, ctypes.byref(t))
         else:
             offs = np.ascontiguousarray(seg_offsets, np.int32)
             mn = np.ascontiguousarray(seg_min, np.float32)
             rg = np.ascontiguousarray(seg_range, np.float32)
-            self._check(self.lib.ocp_autotune(self.h, points.ptr, f3, int(offs[-1]),
-                                              _ptr(offs), _ptr(mn), _ptr(rg), offs.size - 1,
-                                              reps, ctypes.byref(t)))
+            args = (points.ptr, f3, int(offs[-1]), _ptr(offs), _ptr(mn), _ptr(rg),
+                    offs.size - 1, reps, ctypes.byref(t))
+        self.tune_from_cache = False
+        if cache is None:
+            self._check(self.lib.ocp_autotune(self.h, *args))
+        else:
+            hit = ctypes.c_int(0)
+            self._check(self.lib.ocp_autotune_cached(self.h, str(cache).encode(), *args,
+                                                     ctypes.byref(hit)))
+            self.tune_from_cache = bool(hit.value)
         return t.as_dict()
 
     # ---- learned-CDF bucket sort

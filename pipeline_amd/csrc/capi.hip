@@ -194,6 +194,7 @@ struct ocp_context {
   bool stage_timing = false;
   hipEvent_t ev[8] = {};
   hipEvent_t kev[2 * kKernelCount] = {};
+  bool kran[kKernelCount] = {};  // kernels the last timed eager build ran
   bool ev_recorded = false;
   LastBuild last;
   LearnState learn;
@@ -297,7 +298,7 @@ int ensure_workspace(ocp_ctx c, int64_t n, int32_t n_seg, int64_t flag_words) {
     dev_free(w.hist); dev_free(w.counts); dev_free(w.oct_start); dev_free(w.tables);
     dev_free(w.boxes); dev_free(w.box_mm);
     int r = 0;
-    if ((r = dev_alloc(c, &w.hist, int64_t(cap) * 1024)) ||
+    if ((r = dev_alloc(c, &w.hist, int64_t(cap) * hist_stride(10))) ||
         (r = dev_alloc(c, &w.counts, int64_t(cap) * 2)) ||
         (r = dev_alloc(c, &w.oct_start, int64_t(cap) + 1)) ||
         (r = dev_alloc(c, &w.tables, 5 * (int64_t(cap) + 1))) ||
@@ -344,24 +345,29 @@ SegView single_view(int64_t n, float mn, float rg) {
 }
 
 int tune_key(const TuneTable& t) {
-  return ((t.sort_waves * 64 + t.sort_kpt) * 64 + t.morton_ppt) * 64 + t.scan_ipt;
+  const int k = ((t.sort_waves * 64 + t.sort_kpt) * 64 + t.morton_ppt) * 64 + t.scan_ipt;
+  return (k * 2 + (t.sort_bits == 10)) * 32 + t.sort10_kpt;
 }
 
 int64_t flag_words_for(const TuneTable& tt, int64_t n, int32_t n_seg) {
   const int64_t sort_tiles = cdiv(n, sort_tile(tt)) + n_seg;
+  const int64_t sort10_tiles = cdiv(n, 1024 * tt.sort10_kpt) + n_seg;
   const int64_t scan_tiles = cdiv(n, scan_tile(tt)) + n_seg;
-  // scans: tile flags, group sums, tile maxima
-  return std::max(sort_tiles * 256, 3 * scan_tiles);
+  // sort passes: one flag per digit and tile; scans: tile flags, group sums,
+  // tile maxima
+  const int64_t sort = std::max(sort_tiles * 256, tt.sort_bits == 10 ? sort10_tiles * 1024 : 0);
+  return std::max(sort, 3 * scan_tiles);
 }
 
 // Plain plan: one segment, scalar box, no device tables.
-Plan plain_plan(const TuneTable& tt, int64_t n, float mn, float rg) {
+Plan plain_plan(const TuneTable& tt, int64_t n, float mn, float rg, bool perm) {
   Plan p;
   p.n_seg = 1;
   p.n_total = n;
+  p.permutation = perm;
   p.sv = single_view(n, mn, rg);
   p.tiles_morton = static_cast<int32_t>(cdiv(n, morton_tile(tt)));
-  p.tiles_sort = static_cast<int32_t>(cdiv(n, sort_tile(tt)));
+  p.tiles_sort = static_cast<int32_t>(cdiv(n, pipe_sort_tile(tt, perm)));
   p.tiles_scan = static_cast<int32_t>(cdiv(n, scan_tile(tt)));
   p.tiles_node = static_cast<int32_t>(cdiv(n, kNodeTile));
   return p;
@@ -390,7 +396,7 @@ int validate_batch(const int32_t* offsets, const float* smin, const float* srg,
 }
 
 int batched_plan(ocp_ctx c, const int32_t* offsets, const float* smin,
-                 const float* srg, int32_t n_seg, Plan* out) {
+                 const float* srg, int32_t n_seg, bool perm, Plan* out) {
   std::string why;
   if (int r = validate_batch(offsets, smin, srg, n_seg, &why)) return fail(c, r, why);
   const int64_t n = offsets[n_seg];
@@ -406,7 +412,9 @@ int batched_plan(ocp_ctx c, const int32_t* offsets, const float* smin,
   Plan p;
   p.n_seg = n_seg;
   p.n_total = n;
+  p.permutation = perm;
   p.device_boxes = !smin;
+  const int stile = pipe_sort_tile(tt, perm);
   std::vector<int32_t> host(5 * static_cast<size_t>(n_seg + 1));
   int64_t tm = 0, ts = 0, tc = 0, tn = 0;
   for (int32_t k = 0; k <= n_seg; ++k) {
@@ -418,12 +426,13 @@ int batched_plan(ocp_ctx c, const int32_t* offsets, const float* smin,
     if (k < n_seg) {
       const int64_t len = int64_t(offsets[k + 1]) - offsets[k];
       tm += cdiv(len, morton_tile(tt));
-      ts += cdiv(len, sort_tile(tt));
+      ts += cdiv(len, stile);
       tc += cdiv(len, scan_tile(tt));
       tn += cdiv(len, kNodeTile);
     }
   }
-  const bool same = c->tab_tune_key == tune_key(tt) &&
+  const int tab_key = tune_key(tt) * 2 + (pipe_sort_bits(tt, perm) == 10);
+  const bool same = c->tab_tune_key == tab_key &&
                     c->tab_offsets.size() == static_cast<size_t>(n_seg + 1) &&
                     std::equal(offsets, offsets + n_seg + 1, c->tab_offsets.begin());
   if (!same) {
@@ -432,7 +441,7 @@ int batched_plan(ocp_ctx c, const int32_t* offsets, const float* smin,
                                 sizeof(int32_t) * (n_seg + 1), hipMemcpyHostToDevice,
                                 c->stream));
     c->tab_offsets.assign(offsets, offsets + n_seg + 1);
-    c->tab_tune_key = tune_key(tt);
+    c->tab_tune_key = tab_key;
     c->tab_min.clear();
   }
   const bool dev_boxes = !smin;
@@ -476,7 +485,10 @@ struct KScope {
       : c(c_), id(id_), on(c_->stage_timing && !capturing) {
     if (on && c->cold_buf)
       launch_fill_u32(c->cold_buf, ++c->cold_seq, c->cold_words, c->stream);
-    if (on) (void)hipEventRecord(c->kev[2 * id], c->stream);
+    if (on) {
+      (void)hipEventRecord(c->kev[2 * id], c->stream);
+      c->kran[id] = true;
+    }
   }
   ~KScope() {
     if (on) (void)hipEventRecord(c->kev[2 * id + 1], c->stream);
@@ -530,27 +542,33 @@ bool enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
   const TuneTable& tt = c->tune;
   int32_t* n_unique = w.counts;
   int32_t* kalloc = w.counts + w.seg_cap;
+  // 8-bit digits take four passes over the keys, 10-bit digits three; the
+  // Morton kernel writes the buffer that leaves the sorted keys in keys_a.
+  const int bits = pipe_sort_bits(tt, p.permutation);
+  const int passes = bits == 10 ? 3 : 4;
+  if (!capturing)
+    for (bool& r : c->kran) r = false;
   mark(c, 0, capturing);
   // Begin also clears the per-segment U and K counts (unique leaves an
   // empty segment's entries untouched).
   { KScope k(c, kKBegin, capturing);
-    launch_begin(c->ctrl, w.hist, int64_t(p.n_seg) * 1024, s,
+    launch_begin(c->ctrl, w.hist, int64_t(p.n_seg) * hist_stride(bits), s,
                  reinterpret_cast<uint32_t*>(w.counts), int64_t(w.seg_cap) * 2); }
   if (p.device_boxes)
     launch_boxes(pts, f3, p.sv, p.ts_morton, p.tiles_morton, morton_tile(tt), w.box_mm, w.boxes,
                  w.boxes + w.seg_cap, s);
+  uint32_t* src = (passes % 2) ? w.keys_b : w.keys_a;
+  uint32_t* dst = (passes % 2) ? w.keys_a : w.keys_b;
   { KScope k(c, kKMorton, capturing);
-    launch_morton(pts, f3, w.keys_a, p.sv, p.ts_morton, p.tiles_morton, w.hist, tt, s); }
+    launch_morton(pts, f3, src, p.sv, p.ts_morton, p.tiles_morton, w.hist, tt, s, bits); }
   mark(c, 1, capturing);
-  uint32_t* src = w.keys_a;
-  uint32_t* dst = w.keys_b;
   uint32_t* vsrc = nullptr;  // the first pass numbers the points itself
   uint32_t* vdst = w.vals_b;
   // Lookback launches of two builds never run at the same time (each relies
   // on its lowest unfinished tile being resident); the rest may overlap.
   if (c->order_wait && !capturing && c->order_err == hipSuccess)
     c->order_err = hipStreamWaitEvent(s, c->order_wait, 0);
-  for (int pass = 0; pass < 4; ++pass) {
+  for (int pass = 0; pass < passes; ++pass) {
     KScope k(c, kKPass0 + pass, capturing);
     if (p.permutation) {
       // Point indices ride with the keys; every pass is stable, so they end
@@ -562,7 +580,7 @@ bool enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
     } else {
       // The first pass may reorder equal digits: the keys carry no payload.
       launch_onesweep_pass(src, dst, p.sv, p.ts_sort, p.tiles_sort, w.hist, w.flags,
-                           c->ctrl, pass, pass != 0, tt, s);
+                           c->ctrl, pass, pass != 0, tt, s, bits);
     }
     std::swap(src, dst);
   }
@@ -1130,8 +1148,7 @@ int ocp_build(ocp_ctx c, const float* points, int float3, int64_t n, float mn, f
     return fail(c, OCP_ERR_INVALID, "float4 input must be 16-byte aligned");
   const TuneTable& tt = c->tune;
   if (int r = ensure_workspace(c, n, 1, flag_words_for(tt, n, 1))) return r;
-  Plan p = plain_plan(tt, n, mn, rg);
-  p.permutation = c->permutation;
+  Plan p = plain_plan(tt, n, mn, rg, c->permutation);
   if (int r = ensure_perm(c)) return r;
   c->last.valid = true;
   c->last.batched = false;
@@ -1152,7 +1169,7 @@ int ocp_build_autobox(ocp_ctx c, const float* points, int float3, int64_t n) {
     return fail(c, OCP_ERR_INVALID, "float4 input must be 16-byte aligned");
   const TuneTable& tt = c->tune;
   if (int r = ensure_workspace(c, n, 1, flag_words_for(tt, n, 1))) return r;
-  Plan p = plain_plan(tt, n, 0.0f, 1.0f);
+  Plan p = plain_plan(tt, n, 0.0f, 1.0f, c->permutation);
   p.device_boxes = true;
   p.permutation = c->permutation;
   if (int r = ensure_perm(c)) return r;
@@ -1171,7 +1188,7 @@ int ocp_build_batched(ocp_ctx c, const float* points, int float3, const int32_t*
                       const float* smin, const float* srg, int32_t n_seg) {
   if (check_ctx(c)) return OCP_ERR_INVALID;
   Plan p;
-  if (int r = batched_plan(c, offsets, smin, srg, n_seg, &p)) return r;
+  if (int r = batched_plan(c, offsets, smin, srg, n_seg, c->permutation, &p)) return r;
   if (p.n_total > 0 && !points) return fail(c, OCP_ERR_INVALID, "null points");
   p.permutation = c->permutation;
   if (int r = ensure_perm(c)) return r;
@@ -1414,7 +1431,9 @@ int ocp_get_kernel_ms(ocp_ctx c, float* ms, const char** names, int32_t max_kern
   OCP_HIP(c, hipEventSynchronize(c->ev[6]));
   for (int k = 0; k < kKernelCount && k < max_kernels; ++k) {
     if (names) names[k] = kKernelNames[k];
-    if (ms) OCP_HIP(c, hipEventElapsedTime(&ms[k], c->kev[2 * k], c->kev[2 * k + 1]));
+    if (!ms) continue;
+    ms[k] = 0.0f;  // a kernel the build did not launch (a fourth sort pass)
+    if (c->kran[k]) OCP_HIP(c, hipEventElapsedTime(&ms[k], c->kev[2 * k], c->kev[2 * k + 1]));
   }
   return OCP_OK;
 }
@@ -1457,6 +1476,8 @@ int ocp_get_tune(ocp_ctx c, ocp_tune* out) {
   out->morton_ppt = c->tune.morton_ppt;
   out->morton_grid = c->tune.morton_grid;
   out->scan_ipt = c->tune.scan_ipt;
+  out->sort_bits = c->tune.sort_bits;
+  out->sort10_kpt = c->tune.sort10_kpt;
   return OCP_OK;
 }
 
@@ -1465,7 +1486,8 @@ int ocp_set_tune(ocp_ctx c, const ocp_tune* in) {
   const bool sort_ok = sort_config_supported(in->sort_waves, in->sort_kpt) ||
                        (in->sort_waves == 8 && in->sort_kpt == 15);
   if (!sort_ok || !morton_config_supported(in->morton_ppt) ||
-      !scan_config_supported(in->scan_ipt) || in->morton_grid < 1)
+      !scan_config_supported(in->scan_ipt) || in->morton_grid < 1 ||
+      (in->sort_bits != 8 && in->sort_bits != 10) || !sort10_config_supported(in->sort10_kpt))
     return fail(c, OCP_ERR_INVALID, "unsupported tile configuration");
   OCP_HIP(c, hipStreamSynchronize(c->stream));
   c->tune.sort_waves = in->sort_waves;
@@ -1473,6 +1495,8 @@ int ocp_set_tune(ocp_ctx c, const ocp_tune* in) {
   c->tune.morton_ppt = in->morton_ppt;
   c->tune.morton_grid = in->morton_grid;
   c->tune.scan_ipt = in->scan_ipt;
+  c->tune.sort_bits = in->sort_bits;
+  c->tune.sort10_kpt = in->sort10_kpt;
   clear_graphs(c);
   c->tab_offsets.clear();
   return OCP_OK;
@@ -1493,12 +1517,22 @@ int ocp_autotune(ocp_ctx c, const float* points, int float3, int64_t n,
   ocp_get_tune(c, &best);
   float best_ms = 0;
   if (int r = time_cfg(best, &best_ms)) return r;
-  // Coordinate descent: sort tile, then scan tile, then Morton tile/grid.
+  // Coordinate descent: sort tile of 8-bit digits, then the pass count
+  // (four 8-bit passes or three 10-bit ones, two tile sizes), then scan
+  // tile, then Morton tile/grid.
   const int sort_cfgs[][2] = {{8, 15}, {4, 16}, {4, 24}, {8, 8}, {8, 12}, {8, 16}, {16, 8}};
   for (const auto& sc : sort_cfgs) {
     ocp_tune t = best;
     t.sort_waves = sc[0];
     t.sort_kpt = sc[1];
+    float ms;
+    if (int r = time_cfg(t, &ms)) return r;
+    if (ms < best_ms) { best_ms = ms; best = t; }
+  }
+  for (int kpt : {8, 16}) {
+    ocp_tune t = best;
+    t.sort_bits = 10;
+    t.sort10_kpt = kpt;
     float ms;
     if (int r = time_cfg(t, &ms)) return r;
     if (ms < best_ms) { best_ms = ms; best = t; }
@@ -1522,6 +1556,76 @@ int ocp_autotune(ocp_ctx c, const float* points, int float3, int64_t n,
   }
   if (int r = ocp_set_tune(c, &best)) return r;
   if (chosen) *chosen = best;
+  return OCP_OK;
+}
+
+// The workload class a tuning result is kept under: device (architecture,
+// compute units), plain or batched, point layout, and the point and segment
+// counts rounded up to powers of two.
+static std::string tune_class(ocp_ctx c, int float3, int64_t n, int32_t n_seg) {
+  hipDeviceProp_t prop{};
+  std::string arch = "unknown";
+  int cus = 0;
+  if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) {
+    arch = prop.gcnArchName;
+    cus = prop.multiProcessorCount;
+  }
+  const std::string::size_type colon = arch.find(':');
+  if (colon != std::string::npos) arch.resize(colon);
+  auto log2_up = [](int64_t v) {
+    int k = 0;
+    while ((int64_t(1) << k) < v) ++k;
+    return k;
+  };
+  char buf[160];
+  std::snprintf(buf, sizeof(buf), "%s/cu%d/%s/f%d/n2^%d/s2^%d", arch.c_str(), cus,
+                n_seg == 0 ? "plain" : "batched", float3 ? 3 : 4, log2_up(std::max<int64_t>(n, 1)),
+                log2_up(int64_t(std::max(n_seg, 1))));
+  return buf;
+}
+
+int ocp_autotune_cached(ocp_ctx c, const char* cache_path, const float* points, int float3,
+                        int64_t n, const int32_t* offsets, const float* smin, const float* srg,
+                        int32_t n_seg, int reps, ocp_tune* chosen, int* from_cache) {
+  if (check_ctx(c) || !cache_path) return OCP_ERR_INVALID;
+  if (from_cache) *from_cache = 0;
+  int64_t total = n;
+  if (n_seg > 0) {
+    if (!offsets) return fail(c, OCP_ERR_INVALID, "null segment table");
+    total = offsets[n_seg];
+  }
+  const std::string key = tune_class(c, float3, total, n_seg);
+  if (FILE* f = std::fopen(cache_path, "r")) {
+    char line[512];
+    ocp_tune t{}, found{};
+    bool hit = false;
+    while (std::fgets(line, sizeof(line), f)) {
+      char k[256];
+      if (std::sscanf(line, "%255s %d %d %d %d %d %d %d", k, &t.sort_waves, &t.sort_kpt,
+                      &t.morton_ppt, &t.morton_grid, &t.scan_ipt, &t.sort_bits,
+                      &t.sort10_kpt) == 8 &&
+          key == k) {
+        found = t;  // the last entry for the class wins
+        hit = true;
+      }
+    }
+    std::fclose(f);
+    if (hit && ocp_set_tune(c, &found) == OCP_OK) {
+      if (chosen) *chosen = found;
+      if (from_cache) *from_cache = 1;
+      return OCP_OK;
+    }
+  }
+  ocp_tune best{};
+  if (int r = ocp_autotune(c, points, float3, n, offsets, smin, srg, n_seg, reps, &best))
+    return r;
+  if (chosen) *chosen = best;
+  FILE* f = std::fopen(cache_path, "a");
+  if (!f) return fail(c, OCP_ERR_INVALID, std::string("cannot append to ") + cache_path);
+  std::fprintf(f, "%s %d %d %d %d %d %d %d\n", key.c_str(), best.sort_waves, best.sort_kpt,
+               best.morton_ppt, best.morton_grid, best.scan_ipt, best.sort_bits,
+               best.sort10_kpt);
+  std::fclose(f);
   return OCP_OK;
 }
 

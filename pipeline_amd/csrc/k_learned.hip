@@ -149,6 +149,7 @@ __device__ __forceinline__ void load_cells(const LearnTables& tb, int tid, int t
 // be loaded by every tile.
 struct CoarseDigit {
   static constexpr bool kCheap = false;
+  static constexpr int kBits = 8;
   LearnTables tb;
   __device__ __forceinline__ void setup(int32_t, int tid, int threads) const {
     for (int i = tid; i < kLearnCoarse; i += threads) s_cb[i] = tb.fb[i * kLearnFine];
@@ -163,6 +164,7 @@ struct CoarseDigit {
 };
 struct FineDigit {
   static constexpr bool kCheap = false;
+  static constexpr int kBits = 8;
   LearnTables tb;
   __device__ __forceinline__ void setup(int32_t seg, int tid, int threads) const {
     for (int i = tid; i < kLearnFine; i += threads) s_fbs[i] = tb.fb[seg * kLearnFine + i];

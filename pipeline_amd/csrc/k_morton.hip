@@ -6,36 +6,41 @@
 // Persistent workgroups of 256 threads walk a contiguous run of tiles
 // (256 * PPT points each, never straddling a segment). Points arrive as
 // 16-byte float4 loads (or three 4-byte loads for the float3 variant), keys
-// leave as coalesced 4-byte stores. Digit counts accumulate in a 4 x 256 LDS
-// histogram that is flushed with global atomics (zero bins skipped) when the
-// run changes segment and at its end.
+// leave as coalesced 4-byte stores. Digit counts accumulate in an LDS
+// histogram (4 x 256 bins for 8-bit digits, 3 x 1024 for 10-bit ones) that is
+// flushed with global atomics (zero bins skipped) when the run changes
+// segment and at its end.
 #include "ocp_internal.hpp"
 
 namespace ocp {
 namespace {
 
-template <int PPT, bool F3, bool HIST>
+// HB: digit width of the fused histograms (0: none; 8: 4 x 256 bins; 10:
+// 3 x 1024 bins, for keys below 2^30).
+template <int PPT, bool F3, int HB>
 __global__ __launch_bounds__(256) void morton_kernel(
     const float* __restrict__ pts, uint32_t* __restrict__ keys, SegView sv,
     const int32_t* __restrict__ tile_start, int32_t total_tiles,
     uint32_t* __restrict__ hist) {
   constexpr int T = 256 * PPT;
-  __shared__ uint32_t s_hist[HIST ? 1024 : 1];
+  constexpr bool HIST = HB != 0;
+  constexpr int HW = HB == 10 ? 3072 : 1024;  // histogram words per segment
+  __shared__ uint32_t s_hist[HIST ? HW : 1];
   const int tid = threadIdx.x;
   const int32_t per = (total_tiles + gridDim.x - 1) / gridDim.x;
   const int32_t t0 = blockIdx.x * per;
   const int32_t t1 = min(total_tiles, t0 + per);
   if (HIST) {
-    for (int d = tid; d < 1024; d += 256) s_hist[d] = 0;
+    for (int d = tid; d < HW; d += 256) s_hist[d] = 0;
     __syncthreads();
   }
   int32_t cur = -1;
   auto flush = [&](int32_t seg) {
     __syncthreads();
-    for (int d = tid; d < 1024; d += 256) {
+    for (int d = tid; d < HW; d += 256) {
       const uint32_t v = s_hist[d];
       if (v) {
-        atomicAdd(&hist[static_cast<int64_t>(seg) * 1024 + d], v);
+        atomicAdd(&hist[static_cast<int64_t>(seg) * HW + d], v);
         s_hist[d] = 0;
       }
     }
@@ -53,11 +58,15 @@ __global__ __launch_bounds__(256) void morton_kernel(
     auto emit = [&](int64_t g, float x, float y, float z) {
       const uint32_t key = encode(x, y, z, mn, rg);
       keys[g] = key;
-      if (HIST) {
+      if (HB == 8) {
         atomicAdd(&s_hist[key & 255u], 1u);
         atomicAdd(&s_hist[256 + ((key >> 8) & 255u)], 1u);
         atomicAdd(&s_hist[512 + ((key >> 16) & 255u)], 1u);
         atomicAdd(&s_hist[768 + (key >> 24)], 1u);
+      } else if (HB == 10) {
+        atomicAdd(&s_hist[key & 1023u], 1u);
+        atomicAdd(&s_hist[1024 + ((key >> 10) & 1023u)], 1u);
+        atomicAdd(&s_hist[2048 + ((key >> 20) & 1023u)], 1u);
       }
     };
     if (cnt == T) {
@@ -100,21 +109,25 @@ __global__ __launch_bounds__(256) void morton_kernel(
   if (HIST && cur >= 0) flush(cur);
 }
 
+template <int PPT, bool F3>
+void launch_f3(const float* pts, uint32_t* keys, const SegView& sv, const int32_t* tile_start,
+               int32_t total_tiles, uint32_t* hist, int hist_bits, int grid, hipStream_t s) {
+  if (!hist)
+    morton_kernel<PPT, F3, 0><<<grid, 256, 0, s>>>(pts, keys, sv, tile_start, total_tiles, hist);
+  else if (hist_bits == 10)
+    morton_kernel<PPT, F3, 10><<<grid, 256, 0, s>>>(pts, keys, sv, tile_start, total_tiles, hist);
+  else
+    morton_kernel<PPT, F3, 8><<<grid, 256, 0, s>>>(pts, keys, sv, tile_start, total_tiles, hist);
+}
+
 template <int PPT>
 void launch_ppt(const float* pts, bool f3, uint32_t* keys, const SegView& sv,
                 const int32_t* tile_start, int32_t total_tiles, uint32_t* hist,
-                int grid, hipStream_t s) {
-  if (f3) {
-    if (hist)
-      morton_kernel<PPT, true, true><<<grid, 256, 0, s>>>(pts, keys, sv, tile_start, total_tiles, hist);
-    else
-      morton_kernel<PPT, true, false><<<grid, 256, 0, s>>>(pts, keys, sv, tile_start, total_tiles, hist);
-  } else {
-    if (hist)
-      morton_kernel<PPT, false, true><<<grid, 256, 0, s>>>(pts, keys, sv, tile_start, total_tiles, hist);
-    else
-      morton_kernel<PPT, false, false><<<grid, 256, 0, s>>>(pts, keys, sv, tile_start, total_tiles, hist);
-  }
+                int hist_bits, int grid, hipStream_t s) {
+  if (f3)
+    launch_f3<PPT, true>(pts, keys, sv, tile_start, total_tiles, hist, hist_bits, grid, s);
+  else
+    launch_f3<PPT, false>(pts, keys, sv, tile_start, total_tiles, hist, hist_bits, grid, s);
 }
 
 }  // namespace
@@ -126,14 +139,15 @@ bool morton_config_supported(int ppt) {
 void launch_morton(const float* pts, bool float3, uint32_t* keys,
                    const SegView& sv, const int32_t* tile_start,
                    int32_t total_tiles, uint32_t* hist, const TuneTable& tt,
-                   hipStream_t s) {
+                   hipStream_t s, int hist_bits) {
   if (total_tiles <= 0) return;
   const int grid = std::max(1, std::min(total_tiles, tt.morton_grid));
+  const int hb = hist_bits;
   switch (tt.morton_ppt) {
-    case 4: launch_ppt<4>(pts, float3, keys, sv, tile_start, total_tiles, hist, grid, s); break;
-    case 8: launch_ppt<8>(pts, float3, keys, sv, tile_start, total_tiles, hist, grid, s); break;
-    case 32: launch_ppt<32>(pts, float3, keys, sv, tile_start, total_tiles, hist, grid, s); break;
-    default: launch_ppt<16>(pts, float3, keys, sv, tile_start, total_tiles, hist, grid, s); break;
+    case 4: launch_ppt<4>(pts, float3, keys, sv, tile_start, total_tiles, hist, hb, grid, s); break;
+    case 8: launch_ppt<8>(pts, float3, keys, sv, tile_start, total_tiles, hist, hb, grid, s); break;
+    case 32: launch_ppt<32>(pts, float3, keys, sv, tile_start, total_tiles, hist, hb, grid, s); break;
+    default: launch_ppt<16>(pts, float3, keys, sv, tile_start, total_tiles, hist, hb, grid, s); break;
   }
 }
 

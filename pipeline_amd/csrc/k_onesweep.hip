@@ -90,6 +90,21 @@ void launch_cfg(const uint32_t* in, uint32_t* out, const SegView& sv,
         in, out, sv, tile_start, nullptr, hist, pass, flags, ctrl, stage, dig);
 }
 
+// 10-bit digits: one thread per digit, so 16 waves per tile.
+template <int KPT>
+void launch_cfg10(const uint32_t* in, uint32_t* out, const SegView& sv,
+                  const int32_t* tile_start, int32_t total_tiles, const uint32_t* hist,
+                  uint64_t* flags, uint32_t* ctrl, int pass, bool stable, hipStream_t s) {
+  const uint32_t stage = kStagePass0 + static_cast<uint32_t>(pass);
+  const RadixDigit10 dig{10u * static_cast<uint32_t>(pass)};
+  if (stable)
+    onesweep_kernel<16, KPT, RadixDigit10, true><<<total_tiles, 1024, 0, s>>>(
+        in, out, sv, tile_start, nullptr, hist, pass, flags, ctrl, stage, dig);
+  else
+    onesweep_kernel<16, KPT, RadixDigit10, false><<<total_tiles, 1024, 0, s>>>(
+        in, out, sv, tile_start, nullptr, hist, pass, flags, ctrl, stage, dig);
+}
+
 template <int WAVES, int KPT>
 void launch_cfg_kv(const uint32_t* in, uint32_t* out, const uint32_t* vin, uint32_t* vout,
                    const SegView& sv, const int32_t* tile_start, int32_t total_tiles,
@@ -125,6 +140,8 @@ void launch_onesweep_pass_kv(const uint32_t* in, uint32_t* out, const uint32_t* 
                        pass, s);
 }
 
+bool sort10_config_supported(int kpt) { return kpt == 8 || kpt == 16; }
+
 bool sort_config_supported(int waves, int kpt) {
   return (waves == 4 && (kpt == 16 || kpt == 24)) ||
          (waves == 8 && (kpt == 8 || kpt == 12 || kpt == 15 || kpt == 16)) ||
@@ -135,8 +152,15 @@ void launch_onesweep_pass(const uint32_t* in, uint32_t* out,
                           const SegView& sv, const int32_t* tile_start,
                           int32_t total_tiles, const uint32_t* hist,
                           uint64_t* flags, uint32_t* ctrl, int pass, bool stable,
-                          const TuneTable& tt, hipStream_t s) {
+                          const TuneTable& tt, hipStream_t s, int bits) {
   if (total_tiles <= 0) return;
+  if (bits == 10) {
+    if (tt.sort10_kpt == 16)
+      launch_cfg10<16>(in, out, sv, tile_start, total_tiles, hist, flags, ctrl, pass, stable, s);
+    else
+      launch_cfg10<8>(in, out, sv, tile_start, total_tiles, hist, flags, ctrl, pass, stable, s);
+    return;
+  }
 #define OCP_SORT_CFG(W, K)                                                   \
   if (tt.sort_waves == W && tt.sort_kpt == K) {                              \
     launch_cfg<W, K>(in, out, sv, tile_start, total_tiles, hist, flags, ctrl, \

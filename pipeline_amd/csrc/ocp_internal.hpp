@@ -66,14 +66,28 @@ struct TuneTable {
   int morton_ppt = 16;     // points per thread in the Morton kernel
   int morton_grid = 1024;  // persistent Morton workgroups
   int scan_ipt = 16;       // items per thread in unique / edge-scan tiles
+  int sort_bits = 8;       // digit width of a build's sort: 8 (4 passes) or 10 (3)
+  int sort10_kpt = 8;      // keys per thread of a 10-bit pass (16 waves)
 };
 
 inline int sort_tile(const TuneTable& t) { return t.sort_waves * 64 * t.sort_kpt; }
+// Digit width of a build's sort passes: 10-bit digits (three passes over
+// the 30-bit Morton keys) when chosen and no permutation rides along.
+inline int pipe_sort_bits(const TuneTable& t, bool perm) {
+  return (t.sort_bits == 10 && !perm) ? 10 : 8;
+}
+inline int pipe_sort_tile(const TuneTable& t, bool perm) {
+  return pipe_sort_bits(t, perm) == 10 ? 1024 * t.sort10_kpt : sort_tile(t);
+}
+// Digit-count words per segment: four 256-bin histograms for 8-bit digits,
+// three 1024-bin ones for 10-bit digits.
+constexpr int hist_stride(int bits) { return bits == 10 ? 3 * 1024 : 4 * 256; }
 inline int morton_tile(const TuneTable& t) { return 256 * t.morton_ppt; }
 inline int scan_tile(const TuneTable& t) { return 256 * t.scan_ipt; }
 constexpr int kNodeTile = 256;  // BRT nodes per radix-tree / octree tile
 
 bool sort_config_supported(int waves, int kpt);
+bool sort10_config_supported(int kpt);
 bool scan_config_supported(int ipt);
 bool morton_config_supported(int ppt);
 
@@ -106,10 +120,12 @@ void launch_boxes(const float* pts, bool float3, const SegView& sv, const int32_
                   hipStream_t s);
 
 // ---- stage 1 (k_morton.hip)
+// hist (optional): per-segment digit counts of the keys for hist_bits-wide
+// digits (8: four 256-bin histograms; 10: three 1024-bin ones).
 void launch_morton(const float* pts, bool float3, uint32_t* keys,
                    const SegView& sv, const int32_t* tile_start,
                    int32_t total_tiles, uint32_t* hist, const TuneTable& tt,
-                   hipStream_t s);
+                   hipStream_t s, int hist_bits = 8);
 
 // ---- stage 2 (k_onesweep.hip)
 void launch_histogram(const uint32_t* keys, const SegView& sv,
@@ -121,12 +137,14 @@ void launch_histogram_tiles(const uint32_t* keys, const SegView& sv,
                             int tile, uint32_t* hist, const TuneTable& tt,
                             hipStream_t s);
 // stable = false: equal digits may leave in any order (the first pass of a
-// sort of bare keys); full tiles then rank with LDS atomics.
+// sort of bare keys); full tiles then rank with LDS atomics. bits = 10: the
+// pass takes 10-bit digits (16-wave tiles of 1024 * sort10_kpt keys, 1024
+// flags per tile, hist laid out for 10-bit digits).
 void launch_onesweep_pass(const uint32_t* in, uint32_t* out,
                           const SegView& sv, const int32_t* tile_start,
                           int32_t total_tiles, const uint32_t* hist,
                           uint64_t* flags, uint32_t* ctrl, int pass, bool stable,
-                          const TuneTable& tt, hipStream_t s);
+                          const TuneTable& tt, hipStream_t s, int bits = 8);
 
 // Stable pass carrying a value with every key (vin == nullptr: the key's
 // index in its segment).

@@ -7,11 +7,12 @@
 //      lookback makes progress without a shared ticket counter (a single
 //      device-scope counter serialises at one atomic per tile).
 //   2. Keys are loaded wave-contiguously (lane-strided, coalesced) and
-//      ranked per wave with a 64-lane multisplit: eight ballots build the
+//      ranked per wave with a 64-lane multisplit: one ballot per digit bit builds the
 //      mask of lanes holding the same digit; popcount below the lane gives a
 //      stable rank; the lowest lane bumps the wave's LDS digit counter.
-//   3. The tile's 256 digit counts are published at once (aggregate, or
-//      inclusive for a segment's first tile), then each of 256 threads looks
+//   3. The tile's 2^bits digit counts (256, or 1024 for 10-bit digits) are
+//      published at once (aggregate, or inclusive for a segment's first
+//      tile), then each of that many threads looks
 //      back along its digit's chain, kOneSweepWindow tiles per round trip, with
 //      agent-scope loads, and publishes the inclusive prefix.
 //   4. Keys are scattered into LDS in digit order (with their digits, unless
@@ -48,11 +49,12 @@ constexpr int kOneSweepWindow = OCP_ONESWEEP_WINDOW;
 
 // A digit functor: setup(seg, tid, threads) runs once per tile before any
 // digit is taken (per-segment tables go to LDS there); operator() maps a key
-// to its digit in [0, 256).
+// to its digit in [0, 2^kBits).
 struct RadixDigit {
   // A digit this cheap is recomputed from the key in the scatter rather
   // than staged in LDS beside it.
   static constexpr bool kCheap = true;
+  static constexpr int kBits = 8;
   uint32_t shift;
   __device__ __forceinline__ void setup(int32_t, int, int) const {}
   __device__ __forceinline__ uint32_t operator()(uint32_t key) const {
@@ -60,9 +62,20 @@ struct RadixDigit {
   }
 };
 
-// hist: per segment 1024 words; this pass reads words [slot*256, slot*256+256)
-// of its segment. tiles_dev (optional): the number of real tiles, on device,
-// when the grid is sized for an upper bound.
+// 10-bit digits: three passes cover a 30-bit Morton key.
+struct RadixDigit10 {
+  static constexpr bool kCheap = true;
+  static constexpr int kBits = 10;
+  uint32_t shift;
+  __device__ __forceinline__ void setup(int32_t, int, int) const {}
+  __device__ __forceinline__ uint32_t operator()(uint32_t key) const {
+    return (key >> shift) & 1023u;
+  }
+};
+
+// hist: per segment hist_stride(kBits) words; this pass reads the 2^kBits
+// words from slot * 2^kBits of its segment. tiles_dev (optional): the number
+// of real tiles, on device, when the grid is sized for an upper bound.
 template <int WAVES, int KPT, class Digit, bool STABLE = true, bool KV = false>
 __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
     const uint32_t* __restrict__ in, uint32_t* __restrict__ out, SegView sv,
@@ -72,14 +85,18 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
     const uint32_t* __restrict__ vin = nullptr, uint32_t* __restrict__ vout = nullptr) {
   constexpr int THREADS = WAVES * 64;
   constexpr int T = THREADS * KPT;
-  static_assert(THREADS >= 256, "one thread per digit is required");
+  constexpr int BITS = Digit::kBits;
+  constexpr int RADIX = 1 << BITS;
+  constexpr int64_t HSTRIDE = hist_stride(BITS);
+  static_assert(THREADS >= RADIX, "one thread per digit is required");
   static_assert(!KV || STABLE, "a pass that carries values is stable");
   constexpr bool CHEAP = Digit::kCheap;
+  static_assert(CHEAP || BITS == 8, "staged digits are bytes");
   __shared__ uint32_t s_keys[T];
   __shared__ uint32_t s_vals[KV ? T : 1];
   __shared__ uint8_t s_dig[CHEAP ? 1 : T];
-  __shared__ uint32_t s_whist[WAVES * 256];
-  __shared__ int32_t s_dst[256];
+  __shared__ uint32_t s_whist[WAVES * RADIX];
+  __shared__ int32_t s_dst[RADIX];
   __shared__ uint32_t s_tmp[2 * WAVES];
 
   const int32_t t = blockIdx.x;
@@ -90,7 +107,7 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
   const uint32_t w = wave_id();
   const uint32_t lane = lane_id();
   const TileRef r = find_tile(tile_start, sv.n_seg, t);
-  for (int i = tid; i < WAVES * 256; i += THREADS) s_whist[i] = 0;
+  for (int i = tid; i < WAVES * RADIX; i += THREADS) s_whist[i] = 0;
   dig.setup(r.seg, tid, THREADS);
   __syncthreads();
   const int32_t sb = seg_begin(sv, r.seg);
@@ -99,7 +116,7 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
   // The segment's count of this thread's digit, loaded now so that it is in
   // hand when the tile's offsets are formed.
   const uint32_t hv =
-      tid < 256 ? hist[static_cast<int64_t>(r.seg) * 1024 + hist_slot * 256 + tid] : 0u;
+      tid < RADIX ? hist[static_cast<int64_t>(r.seg) * HSTRIDE + hist_slot * RADIX + tid] : 0u;
 
   uint32_t keys[KPT];
   uint32_t dg[KPT];
@@ -132,13 +149,30 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
 #pragma unroll
     for (int k = 0; k < KPT; ++k) {
       const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
-      dg[k] = idx < cnt ? dig(keys[k]) : 255u;  // padding keys rank last
+      dg[k] = idx < cnt ? dig(keys[k]) : uint32_t(RADIX - 1);  // padding keys rank last
     }
+  }
+
+  // A segment whose keys all share this pass's digit (one bin of its
+  // histogram holds every key) needs no ranking: a stable pass over it is the
+  // identity, so its tiles copy their keys through. Every tile of the segment
+  // decides the same way, so none waits on a flag that is never published.
+  const int32_t seg_len = seg_end(sv, r.seg) - sb;
+  if (__syncthreads_or(tid < RADIX && static_cast<int32_t>(hv) == seg_len)) {
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      const int32_t idx = lbase + k * 64;
+      if (idx < cnt) {
+        out[base + idx] = keys[k];
+        if constexpr (KV) vout[base + idx] = vals[k];
+      }
+    }
+    return;
   }
 
   // Wave-level multisplit. The read of the digit counter by every lane of a
   // group happens in the same LDS instruction, before the leader's update.
-  uint32_t* wh = s_whist + w * 256;
+  uint32_t* wh = s_whist + w * RADIX;
   if (STABLE || cnt < T) {
     const uint64_t lt = lanemask_lt();
 #pragma unroll
@@ -146,7 +180,7 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
       const uint32_t d = dg[k];
       uint64_t m = ~0ull;
 #pragma unroll
-      for (int b = 0; b < 8; ++b) {
+      for (int b = 0; b < BITS; ++b) {
         const bool bit = (d >> b) & 1u;
         const uint64_t bal = __ballot(bit);
         m &= bit ? bal : ~bal;
@@ -168,34 +202,34 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
 
   // Per-digit: exclusive prefix over waves, tile count, publish.
   uint32_t tile_cnt = 0;
-  if (tid < 256) {
+  if (tid < RADIX) {
     uint32_t run = 0;
 #pragma unroll
     for (int ww = 0; ww < WAVES; ++ww) {
-      const uint32_t c = s_whist[ww * 256 + tid];
-      s_whist[ww * 256 + tid] = run;
+      const uint32_t c = s_whist[ww * RADIX + tid];
+      s_whist[ww * RADIX + tid] = run;
       run += c;
     }
-    if (tid == 255) run -= static_cast<uint32_t>(T - cnt);  // padding keys
+    if (tid == RADIX - 1) run -= static_cast<uint32_t>(T - cnt);  // padding keys
     tile_cnt = run;
-    flag_store(&flags[static_cast<int64_t>(t) * 256 + tid],
+    flag_store(&flags[static_cast<int64_t>(t) * RADIX + tid],
                flag_pack(tag, r.local == 0 ? kFlagInc : kFlagAgg, run));
   }
   // Tile-local digit starts and the segment's digit starts.
   uint32_t dstart, hex;
   block_excl_scan2<THREADS>(tile_cnt, hv, s_tmp, &dstart, &hex);
-  if (tid < 256) {
+  if (tid < RADIX) {
     // Each wave's offset of this digit becomes its LDS position in the
     // tile: digit start + keys of the digit in earlier waves.
 #pragma unroll
-    for (int ww = 0; ww < WAVES; ++ww) s_whist[ww * 256 + tid] += dstart;
+    for (int ww = 0; ww < WAVES; ++ww) s_whist[ww * RADIX + tid] += dstart;
     uint32_t prev = 0;
     if (r.local > 0) {
       bool late = false;
-      prev = lookback_n<kOneSweepWindow>(flags + tid, 256, t, t - r.local, tag,
+      prev = lookback_n<kOneSweepWindow>(flags + tid, RADIX, t, t - r.local, tag,
                                          wait_limit(ctrl), &late);
       if (late) raise_timeout(ctrl, tag);
-      flag_store(&flags[static_cast<int64_t>(t) * 256 + tid],
+      flag_store(&flags[static_cast<int64_t>(t) * RADIX + tid],
                  flag_pack(tag, kFlagInc, prev + tile_cnt));
     }
     s_dst[tid] = sb + static_cast<int32_t>(hex + prev) - static_cast<int32_t>(dstart);

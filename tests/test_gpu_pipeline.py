@@ -204,6 +204,7 @@ def test_tuned_configs_give_identical_outputs(ctx):
     base = ctx.get_tune()
     configs = ([dict(sort_waves=w, sort_kpt=k) for w, k in
                 ((8, 15), (4, 16), (4, 24), (8, 8), (8, 12), (8, 16), (16, 8))] +
+               [dict(sort_bits=10, sort10_kpt=k) for k in (8, 16)] +
                [dict(scan_ipt=i) for i in (8, 16, 32)] +
                [dict(morton_ppt=p, morton_grid=g) for p in (4, 8, 16, 32) for g in (512, 2048)])
     try:
@@ -220,3 +221,67 @@ def test_tuned_configs_give_identical_outputs(ctx):
         ctx.set_tune(**base)
         d.free()
         db.free()
+
+
+@pytest.mark.parametrize("kpt", [8, 16])
+def test_ten_bit_digits_full_size(ctx, kpt):
+    """Three passes of 10-bit digits (the autotuner's other pass count) give
+    the oracle's bytes on the 10M workload, on a batch whose clouds span many
+    tiles (eager and from a graph), and leave builds with the permutation on
+    four stable 8-bit passes."""
+    base = ctx.get_tune()
+    pts = clouds.uniform(10_000_000, seed=114514)
+    bpts, offs, smin, srg, _ = _batched_case(12, 4096, 2_000_000, seed=21)
+    d = ctx.upload(pts)
+    db = ctx.upload(bpts)
+    try:
+        ctx.set_tune(**{**base, "sort_bits": 10, "sort10_kpt": kpt})
+        ctx.build(d, pts.shape[0], 0.0, 1024.0)
+        _check_segment(ctx.fetch(), 0, oracle.build_plain(pts))
+        exps = [oracle.build_plain(bpts[offs[s]:offs[s + 1]], float(smin[s]), float(srg[s]))
+                for s in range(offs.size - 1)]
+        for graph in (False, True):
+            ctx.set_graph_mode(graph)
+            for _ in range(2):
+                ctx.build_batched(db, offs, smin, srg)
+                out = ctx.fetch(offs)
+                for s in range(offs.size - 1):
+                    _check_segment(out, s, exps[s])
+        ctx.set_graph_mode(False)
+        small = pts[:300000]
+        ds = ctx.upload(small)
+        ctx.set_permutation(True)
+        ctx.build(ds, small.shape[0], 0.0, 1024.0)
+        out = ctx.fetch()
+        keys = oracle.morton(small, 0.0, 1024.0)
+        np.testing.assert_array_equal(out.sorted_index[0], np.argsort(keys, kind="stable"))
+        _check_segment(out, 0, oracle.build_plain(small))
+    finally:
+        ctx.set_permutation(False)
+        ctx.set_graph_mode(False)
+        ctx.set_tune(**base)
+        d.free()
+        db.free()
+
+
+def test_autotune_cache(ctx, tmp_path):
+    """The tuning cache: a first call tunes (pass count included) and appends
+    the workload class; a second call applies it without timing; the build
+    stays exact under the chosen configuration."""
+    pts = clouds.uniform(200_000, seed=3)
+    d = ctx.upload(pts)
+    base = ctx.get_tune()
+    path = tmp_path / "tune.txt"
+    try:
+        t1 = ctx.autotune(d, reps=1, cache=str(path))
+        assert not ctx.tune_from_cache and t1["sort_bits"] in (8, 10)
+        ctx.set_tune(**base)
+        t2 = ctx.autotune(d, reps=1, cache=str(path))
+        assert ctx.tune_from_cache and t2 == t1 and ctx.get_tune() == t1
+        lines = path.read_text().splitlines()
+        assert len(lines) == 1 and "/plain/f4/n2^18/" in lines[0]
+        ctx.build(d, pts.shape[0], 0.0, 1024.0)
+        _check_segment(ctx.fetch(), 0, oracle.build_plain(pts))
+    finally:
+        ctx.set_tune(**base)
+        d.free()

@@ -60,6 +60,28 @@ def test_sort_random(ctx, n):
     np.testing.assert_array_equal(a.to_host(), np.sort(keys))
 
 
+def test_sort_constant_digits_copy_through(ctx):
+    """Passes whose digit is the same for every key copy the keys (and their
+    values) through: the result is still the sort and the stable argsort."""
+    rng = np.random.default_rng(9)
+    cases = (np.uint32(0xAB120000) | rng.integers(0, 1 << 16, 300001).astype(np.uint32),
+             (rng.integers(0, 256, 100000).astype(np.uint32) << np.uint32(8)) |
+             np.uint32(0x5A0000AB))
+    for keys in cases:
+        n = keys.size
+        a = ctx.upload(keys)
+        b = ctx.empty(n, np.uint32)
+        ctx.k_SortKeys(a, b, n)
+        np.testing.assert_array_equal(a.to_host(), np.sort(keys))
+        k = ctx.upload(keys)
+        v = ctx.upload(np.arange(n, dtype=np.uint32))
+        ka = ctx.empty(n, np.uint32)
+        va = ctx.empty(n, np.uint32)
+        ctx.k_SortPairs(k, v, ka, va, n)
+        np.testing.assert_array_equal(k.to_host(), np.sort(keys))
+        np.testing.assert_array_equal(v.to_host(), np.argsort(keys, kind="stable"))
+
+
 def test_sort_kat_and_duplicates(ctx):
     # tests/test_simple_radix_sort_cpu.cpp + tests/xmake.lua:9-13: 10..1 -> 1..10
     kat = np.arange(10, 0, -1, dtype=np.uint32)
