@@ -376,6 +376,8 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
+    ap.add_argument("--no-cold", action="store_true",
+                    help="skip the per-kernel run with a cache flush before every kernel")
     ap.add_argument("--tune-cache", default=None,
                     help="tuning cache file: reuse a tuned workload class instead of timing")
     ap.add_argument("--learned-frames", type=int, default=4,
@@ -431,7 +433,7 @@ def main() -> int:
 
     print("per-kernel timing: warm, cold", file=sys.stderr, flush=True)
     kmean = per_kernel(False)
-    kcold = per_kernel(True)
+    kcold = {} if args.no_cold else per_kernel(True)
     u_arr, k_arr = ctx.counts(1)
     u, k = int(u_arr[0]), int(k_arr[0])
 
