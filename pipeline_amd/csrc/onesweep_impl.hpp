@@ -98,6 +98,7 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
   __shared__ uint32_t s_whist[WAVES * RADIX];
   __shared__ int32_t s_dst[RADIX];
   __shared__ uint32_t s_tmp[2 * WAVES];
+  __shared__ uint32_t s_trivial;
 
   const int32_t t = blockIdx.x;
   if (tiles_dev && t >= *tiles_dev) return;
@@ -108,6 +109,7 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
   const uint32_t lane = lane_id();
   const TileRef r = find_tile(tile_start, sv.n_seg, t);
   for (int i = tid; i < WAVES * RADIX; i += THREADS) s_whist[i] = 0;
+  if (tid == 0) s_trivial = 0u;
   dig.setup(r.seg, tid, THREADS);
   __syncthreads();
   const int32_t sb = seg_begin(sv, r.seg);
@@ -154,21 +156,11 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
   }
 
   // A segment whose keys all share this pass's digit (one bin of its
-  // histogram holds every key) needs no ranking: a stable pass over it is the
-  // identity, so its tiles copy their keys through. Every tile of the segment
-  // decides the same way, so none waits on a flag that is never published.
-  const int32_t seg_len = seg_end(sv, r.seg) - sb;
-  if (__syncthreads_or(tid < RADIX && static_cast<int32_t>(hv) == seg_len)) {
-#pragma unroll
-    for (int k = 0; k < KPT; ++k) {
-      const int32_t idx = lbase + k * 64;
-      if (idx < cnt) {
-        out[base + idx] = keys[k];
-        if constexpr (KV) vout[base + idx] = vals[k];
-      }
-    }
-    return;
-  }
+  // histogram holds every key) needs no pass: a stable pass over it is the
+  // identity. The thread of that bin flags it; after the barrier that
+  // follows the ranking, the tile copies its keys through. Every tile of the
+  // segment decides the same way, so none waits on a flag never published.
+  if (tid < RADIX && static_cast<int32_t>(hv) == seg_end(sv, r.seg) - sb) s_trivial = 1u;
 
   // Wave-level multisplit. The read of the digit counter by every lane of a
   // group happens in the same LDS instruction, before the leader's update.
@@ -199,6 +191,17 @@ __global__ __launch_bounds__(WAVES * 64) void onesweep_kernel(
                                       __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   __syncthreads();
+  if (s_trivial) {
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      const int32_t idx = lbase + k * 64;
+      if (idx < cnt) {
+        out[base + idx] = keys[k];
+        if constexpr (KV) vout[base + idx] = vals[k];
+      }
+    }
+    return;
+  }
 
   // Per-digit: exclusive prefix over waves, tile count, publish.
   uint32_t tile_cnt = 0;
