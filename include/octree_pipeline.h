@@ -231,6 +231,22 @@ int ocp_build_batched(ocp_ctx ctx, const float* points, int float3,
  * NULL). */
 int ocp_check_batch(const int32_t* seg_offsets, const float* seg_min,
                     const float* seg_range, int32_t n_seg);
+/* Device memory a context allocates, without a context or device: for a
+ * first build of n points in n_seg clouds (n_seg 0: plain) with this tile
+ * table (NULL: the defaults) and the permutation on or off, and the scratch
+ * of the per-stage sort, unique and scan entries on n entries (the
+ * reference's OneSweepData and friends, which its caller allocates,
+ * include/shared/types.h:42-65). A build can grow the node buffer later if
+ * its octree needs more than n slots. */
+typedef struct ocp_workspace_sizes {
+  int64_t build_bytes;   /* ocp_build / ocp_build_batched workspace */
+  int64_t sort_bytes;    /* ocp_k_SortKeys, ocp_k_DigitBinning, ocp_k_SortPairs */
+  int64_t unique_bytes;  /* ocp_k_CountUnique, ocp_k_CountUniquePairs */
+  int64_t scan_bytes;    /* ocp_k_PartialSum, ocp_k_EdgeCount */
+} ocp_workspace_sizes;
+struct ocp_tune;
+int ocp_workspace_size(const struct ocp_tune* tune, int64_t n, int32_t n_seg,
+                       int permutation, ocp_workspace_sizes* out);
 /* ocp_build with the cloud's cube box reduced on the device (as for
  * ocp_build_batched with NULL boxes): no host value is needed before the
  * build. The reference fixes the box at [0, 1024) (benchmarks/config.hpp:

@@ -78,6 +78,14 @@ class LearnedStats(ctypes.Structure):
         return {f: int(getattr(self, f)) for f, _ in self._fields_ if f != "reserved"}
 
 
+class WorkspaceSizes(ctypes.Structure):
+    _fields_ = [("build_bytes", _I64), ("sort_bytes", _I64), ("unique_bytes", _I64),
+                ("scan_bytes", _I64)]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
 class TreeReport(ctypes.Structure):
     """ocp_tree_report: counted violations of the octree's structure, plus
     the node-mask links and those whose child lies in the parent's octant."""
@@ -140,6 +148,7 @@ SIGNATURES = {
     "ocp_build_batched": (_INT, [_P, _P, _INT, _P, _P, _P, _I32]),
     "ocp_build_autobox": (_INT, [_P, _P, _INT, _I64]),
     "ocp_check_batch": (_INT, [_P, _P, _P, _I32]),
+    "ocp_workspace_size": (_INT, [_P, _I64, _I32, _INT, ctypes.POINTER(WorkspaceSizes)]),
     "ocp_set_graph_mode": (_INT, [_P, _INT]),
     "ocp_get_results": (_INT, [_P, ctypes.POINTER(Results)]),
     "ocp_get_counts": (_INT, [_P, _P, _P]),
@@ -202,6 +211,20 @@ def load_library(path: str = LIB_PATH):
 
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def workspace_size(n: int, n_seg: int = 0, permutation: bool = False, tune: dict | None = None):
+    """Device bytes a context allocates for a build of n points (and the
+    per-stage sort, unique and scan scratch); no GPU used."""
+    out = WorkspaceSizes()
+    t = None
+    if tune is not None:
+        t = Tune(**{k: int(v) for k, v in tune.items()})
+    rc = load_library().ocp_workspace_size(ctypes.byref(t) if t is not None else None, int(n),
+                                           int(n_seg), int(bool(permutation)), ctypes.byref(out))
+    if rc != OCP_OK:
+        raise OcpError(f"ocp_workspace_size: status {rc}")
+    return out.as_dict()
 
 
 def check_batch(seg_offsets, seg_min, seg_range) -> int:

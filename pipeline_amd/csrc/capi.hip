@@ -1211,6 +1211,45 @@ int ocp_build_batched(ocp_ctx c, const float* points, int float3, const int32_t*
                             &c->last.offsets, c->tune));
 }
 
+int ocp_workspace_size(const ocp_tune* tune, int64_t n, int32_t n_seg, int permutation,
+                       ocp_workspace_sizes* out) {
+  if (!out || n < 0 || n > kMaxPoints || n_seg < 0) return OCP_ERR_INVALID;
+  TuneTable tt;
+  if (tune) {
+    tt.sort_waves = tune->sort_waves;
+    tt.sort_kpt = tune->sort_kpt;
+    tt.morton_ppt = tune->morton_ppt;
+    tt.morton_grid = tune->morton_grid;
+    tt.scan_ipt = tune->scan_ipt;
+    tt.sort_bits = tune->sort_bits;
+    tt.sort10_kpt = tune->sort10_kpt;
+    if (!sort_config_supported(tt.sort_waves, tt.sort_kpt) && !(tt.sort_waves == 8 && tt.sort_kpt == 15))
+      return OCP_ERR_INVALID;
+    if (!scan_config_supported(tt.scan_ipt) || !sort10_config_supported(tt.sort10_kpt))
+      return OCP_ERR_INVALID;
+  }
+  // The allocations of ensure_workspace, alloc_nodes and ensure_perm for a
+  // first build of this shape, plus the context's control words and digit
+  // counts (ocp_create).
+  const int64_t cap = std::max<int64_t>(n, 4096);
+  const int64_t seg = std::max<int32_t>(n_seg, 1);
+  int64_t b = 2 * 4 * cap          // keys_a, keys_b
+              + 4 * 1 * cap        // prefix_n, hl, hr, parent prefix lengths
+              + 4 * 4 * cap        // left child, parent, edge counts, offsets
+              + cap * int64_t(sizeof(OctNode)) + 16;  // nodes (K_cap starts at N)
+  b += 4 * (seg * hist_stride(10) + 2 * seg + (seg + 1) + 6 * (seg + 1) + 4 * seg);
+  b += 8 * std::max<int64_t>(flag_words_for(tt, n, static_cast<int32_t>(seg)), 1 << 16);
+  if (permutation) b += 2 * 4 * cap;
+  b += 4 * (kCtrlWords + 1024);
+  out->build_bytes = b;
+  const int64_t sort_tiles = cdiv(n, sort_tile(tt));
+  const int64_t scan_tiles = cdiv(n, scan_tile(tt));
+  out->sort_bytes = 8 * std::max<int64_t>(sort_tiles * 256, 1 << 16);
+  out->unique_bytes = 8 * std::max<int64_t>(2 * scan_tiles, 1 << 16);
+  out->scan_bytes = out->unique_bytes;
+  return OCP_OK;
+}
+
 int ocp_check_batch(const int32_t* offsets, const float* smin, const float* srg,
                     int32_t n_seg) {
   std::string why;

@@ -132,3 +132,16 @@ def test_fixture_matches_fused_build_and_oracle(tmp_path, n):
     exp = oracle.build_plain(pts)
     u, k = (int(x) for x in r.stdout.split())
     assert (u, k) == (exp["n_unique"], exp["n_oct"])
+
+
+def test_workspace_size_query():
+    """The workspace query needs no device and grows with the workload."""
+    a = pipeline_amd.workspace_size(1_000_000)
+    b = pipeline_amd.workspace_size(10_000_000)
+    c = pipeline_amd.workspace_size(10_000_000, permutation=True)
+    d = pipeline_amd.workspace_size(10_000_000, 512)
+    assert 0 < a["build_bytes"] < b["build_bytes"] < c["build_bytes"]
+    assert b["build_bytes"] < d["build_bytes"]
+    assert a["sort_bytes"] <= b["sort_bytes"] and b["scan_bytes"] > 0
+    with pytest.raises(pipeline_amd.OcpError):
+        pipeline_amd.workspace_size(-1)
