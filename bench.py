@@ -30,10 +30,11 @@ Also reported (same JSON line):
                 each with its own box) built by one segmented launch per stage
                 from a hipGraph, against 512 sequential plain builds (each
                 replayed from its own graph) of the same clouds;
-  learned_sort  2^26 clustered points over successive frames (rigid motion
-                <= 0.5 deg, <= 2 units): the CDF model is trained on 1% key
-                samples of the earlier frames and sorts frame t; OneSweep
-                sorts the same keys; both outputs are compared.
+  learned_sort  2^26 clustered points over 30 successive frames (rigid motion
+                <= 0.5 deg, <= 2 units, applied on the GPU): the CDF model is
+                trained on 1% key samples of the earlier frames and sorts
+                frame t; OneSweep sorts the same keys; both outputs are
+                compared.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload plain|clustered]
                   [--learned-frames F] [--no-batched] [--no-learned]
@@ -302,23 +303,36 @@ def bench_stream(pipeline_amd, clouds, n: int, frames: int = 32):
     return out
 
 
+def _motion(seed: int, center: np.ndarray, max_deg: float = 0.5, max_shift: float = 2.0):
+    """The 3x4 [R | t] of clouds.rigid_motion's step (same draws from the
+    same seed): rotation by <= max_deg about a random axis through center,
+    then a shift <= max_shift per axis; returns (affine, shift)."""
+    rng = np.random.default_rng(seed)
+    axis = rng.normal(size=3)
+    axis /= np.linalg.norm(axis)
+    ang = np.deg2rad(rng.uniform(-max_deg, max_deg))
+    k = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    rot = np.eye(3) + np.sin(ang) * k + (1 - np.cos(ang)) * (k @ k)
+    shift = rng.uniform(-max_shift, max_shift, 3)
+    return np.hstack([rot, (center + shift - rot @ center)[:, None]]), shift
+
+
 def bench_learned(ctx, clouds, frames: int, n: int = 1 << 26, reps: int = 5):
-    """Learned-bucket sort vs OneSweep on frames of a moving clustered cloud."""
-    cur = clouds.clustered(n, seed=114514)
-    keys = []
-    for t in range(frames):
-        if t:
-            cur = clouds.rigid_motion(cur, seed=1000 + t)
-        dp = ctx.upload(cur)
-        k = ctx.empty(n, np.uint32)
-        ctx.k_ComputeMortonCode(dp, k, n, 0.0, 1024.0)
-        ctx.synchronize()
-        dp.free()
-        keys.append(k)
-    del cur
+    """Learned-bucket sort vs OneSweep over successive frames of a moving
+    clustered cloud. Each frame moves on the device (ocp_transform_points,
+    the rotation centre carried forward by the shifts, clipping aside) and
+    its keys come from the pipeline's Morton kernel. Frame t is sorted with a
+    model trained on 1% key samples of frames < t."""
+    host = clouds.clustered(n, seed=114514)
+    center = host[:, :3].astype(np.float64).mean(axis=0)
+    pts = ctx.upload(host)
+    del host
+    hi = float(np.nextafter(np.float32(1024.0), np.float32(0.0)))
     work = ctx.empty(n, np.uint32)
     alt = ctx.empty(n, np.uint32)
     ref = ctx.empty(n, np.uint32)
+    keys_prev = ctx.empty(n, np.uint32)
+    keys_cur = ctx.empty(n, np.uint32)
 
     def timed(fn, src):
         ts = []
@@ -331,30 +345,38 @@ def bench_learned(ctx, clouds, frames: int, n: int = 1 << 26, reps: int = 5):
             ts.append(time.perf_counter() - t0)
         return float(np.median(ts[1:])) * 1e3
 
+    ctx.k_ComputeMortonCode(pts, keys_prev, n, 0.0, 1024.0)
     ctx.cdf_clear_samples()
     per = []
     for t in range(1, frames):
-        ctx.cdf_add_samples(keys[t - 1], n, 100)     # 1% samples of frame t-1
+        affine, shift = _motion(1000 + t, center)
+        center = center + shift
+        ctx.transform_points(pts, pts, affine, 0.0, hi)
+        ctx.k_ComputeMortonCode(pts, keys_cur, n, 0.0, 1024.0)
+        ctx.cdf_add_samples(keys_prev, n, 100)     # 1% samples of frame t-1
+        ctx.synchronize()
         t0 = time.perf_counter()
         loss = ctx.cdf_train(steps=2000, lr=1.0, momentum=0.9, seed=t)
         train_ms = (time.perf_counter() - t0) * 1e3
-        ms_os = timed(lambda: ctx.k_SortKeys(work, alt, n), keys[t])
+        ms_os = timed(lambda: ctx.k_SortKeys(work, alt, n), keys_cur)
         ctx.copy(ref, work, 4 * n)
         st = {}
-        ms_l = timed(lambda: st.update(ctx.sort_learned(work, alt, n)), keys[t])
+        ms_l = timed(lambda: st.update(ctx.sort_learned(work, alt, n)), keys_cur)
         exact = bool(np.array_equal(work.to_host(), ref.to_host()))
         per.append({"frame": t, "onesweep_ms": round(ms_os, 4), "learned_ms": round(ms_l, 4),
                     "exact": exact, "train_ms": round(train_ms, 1), "loss": loss,
                     "keys_fallback": st["keys_big"], "max_bucket": st["max_bucket"]})
+        keys_prev, keys_cur = keys_cur, keys_prev
     ctx.cdf_reset(0, (1 << 30) - 1)
-    ms_lin = timed(lambda: ctx.sort_learned(work, alt, n), keys[-1])
-    for a in keys + [work, alt, ref]:
+    ms_lin = timed(lambda: ctx.sort_learned(work, alt, n), keys_prev)
+    for a in (pts, work, alt, ref, keys_prev, keys_cur):
         a.free()
     os_ms = float(np.mean([p["onesweep_ms"] for p in per]))
     l_ms = float(np.mean([p["learned_ms"] for p in per]))
     return {
-        "n": n, "frames": frames, "motion": "rotation <= 0.5 deg about a random axis, "
-        "shift <= 2 units per axis, per frame", "samples": "every 100th key of each earlier frame",
+        "n": n, "frames": frames, "motion": "rotation <= 0.5 deg about a random axis through "
+        "the centre, shift <= 2 units per axis, per frame, applied on the GPU",
+        "samples": "every 100th key of each earlier frame (the last 2^22 kept)",
         "onesweep_ms": round(os_ms, 4), "learned_ms": round(l_ms, 4),
         "speedup_vs_onesweep": round(os_ms / l_ms, 3),
         "untrained_linear_cdf_ms": round(ms_lin, 4),
@@ -380,8 +402,8 @@ def main() -> int:
                     help="skip the per-kernel run with a cache flush before every kernel")
     ap.add_argument("--tune-cache", default=None,
                     help="tuning cache file: reuse a tuned workload class instead of timing")
-    ap.add_argument("--learned-frames", type=int, default=4,
-                    help="frames of the learned-sort comparison (30 for the full sequence)")
+    ap.add_argument("--learned-frames", type=int, default=30,
+                    help="frames of the learned-sort comparison (the full 30-frame sequence)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "

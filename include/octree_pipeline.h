@@ -252,6 +252,13 @@ int ocp_workspace_size(const struct ocp_tune* tune, int64_t n, int32_t n_seg,
  * build. The reference fixes the box at [0, 1024) (benchmarks/config.hpp:
  * 7-10). */
 int ocp_build_autobox(ocp_ctx ctx, const float* points, int float3, int64_t n);
+/* Moves a cloud in device memory: p' = clamp(A p + t, lo, hi) per coordinate,
+ * with affine = {A row-major (9 floats), t (3 floats)}; a float4 point's
+ * fourth value passes through; src == dst is allowed. Successive frames of a
+ * moving sensor are made this way (rigid motion: A a rotation). No
+ * counterpart in the reference, whose frames are files on disk. */
+int ocp_transform_points(ocp_ctx ctx, const float* src, float* dst, int float3, int64_t n,
+                         const float* affine, float lo, float hi);
 /* Builds carry the key->point permutation when enabled: every sort pass
  * moves a point index with its key (and is stable), unique keeps each key's
  * first point, and ocp_results gives both (sorted_index, unique_index).

@@ -147,6 +147,7 @@ SIGNATURES = {
     "ocp_build": (_INT, [_P, _P, _INT, _I64, _F, _F]),
     "ocp_build_batched": (_INT, [_P, _P, _INT, _P, _P, _P, _I32]),
     "ocp_build_autobox": (_INT, [_P, _P, _INT, _I64]),
+    "ocp_transform_points": (_INT, [_P, _P, _P, _INT, _I64, _P, _F, _F]),
     "ocp_check_batch": (_INT, [_P, _P, _P, _I32]),
     "ocp_workspace_size": (_INT, [_P, _I64, _I32, _INT, ctypes.POINTER(WorkspaceSizes)]),
     "ocp_set_graph_mode": (_INT, [_P, _INT]),
@@ -502,6 +503,17 @@ class Context:
         n = points.shape[0] if n is None else n
         f3 = int(points.shape[-1] == 3)
         self._check(self.lib.ocp_build_autobox(self.h, points.ptr, f3, int(n)))
+
+    def transform_points(self, src: DeviceArray, dst: DeviceArray, affine, lo: float,
+                         hi: float, n: int | None = None):
+        """dst = clamp(A src + t, lo, hi) per point on the device; affine is
+        a 3x4 [A | t] (the fourth float of a float4 point passes through)."""
+        n = src.shape[0] if n is None else n
+        m = np.ascontiguousarray(np.asarray(affine, np.float64).reshape(3, 4), np.float64)
+        flat = np.concatenate([m[:, :3].ravel(), m[:, 3]]).astype(np.float32)
+        self._check(self.lib.ocp_transform_points(self.h, src.ptr, dst.ptr,
+                                                  int(src.shape[-1] == 3), int(n), _ptr(flat),
+                                                  float(lo), float(hi)))
 
     def boxes(self):
         """(seg_min, seg_range) host copies of the boxes the last batched or

@@ -1184,6 +1184,20 @@ int ocp_build_autobox(ocp_ctx c, const float* points, int float3, int64_t n) {
                   graph_key("plain-autobox", points, float3, n, 0, 0, nullptr, tt));
 }
 
+int ocp_transform_points(ocp_ctx c, const float* src, float* dst, int float3, int64_t n,
+                         const float* affine, float lo, float hi) {
+  if (check_ctx(c)) return OCP_ERR_INVALID;
+  if (n < 0) return fail(c, OCP_ERR_INVALID, "n < 0");
+  if (!affine) return fail(c, OCP_ERR_INVALID, "null affine");
+  if (n > 0 && (!src || !dst)) return fail(c, OCP_ERR_INVALID, "null points");
+  if (!(lo <= hi)) return fail(c, OCP_ERR_INVALID, "lo > hi");
+  Affine m;
+  std::memcpy(m.m, affine, sizeof(m.m));
+  launch_transform(src, dst, n, float3 != 0, m, lo, hi, c->stream);
+  OCP_CHECK_LAUNCH(c);
+  return OCP_OK;
+}
+
 int ocp_build_batched(ocp_ctx c, const float* points, int float3, const int32_t* offsets,
                       const float* smin, const float* srg, int32_t n_seg) {
   if (check_ctx(c)) return OCP_ERR_INVALID;

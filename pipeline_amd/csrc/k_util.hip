@@ -99,7 +99,38 @@ __global__ __launch_bounds__(256) void copy_kernel(uint4* __restrict__ dst,
     dst[i] = src[i];
 }
 
+// p' = clamp(A p + t, lo, hi) for each point (A row-major 3x3, t in m[9..11]);
+// the fourth float of a float4 point passes through. In place is allowed.
+template <int S>
+__global__ __launch_bounds__(256) void transform_kernel(const float* __restrict__ src,
+                                                        float* __restrict__ dst, int64_t n,
+                                                        Affine m, float lo, float hi) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += stride) {
+    const float* p = src + i * S;
+    const float x = p[0], y = p[1], z = p[2];
+    float* q = dst + i * S;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const float v = fmaf(m.m[3 * r], x, fmaf(m.m[3 * r + 1], y, fmaf(m.m[3 * r + 2], z, m.m[9 + r])));
+      q[r] = fminf(fmaxf(v, lo), hi);
+    }
+    if (S == 4) q[3] = p[3];
+  }
+}
+
 }  // namespace
+
+void launch_transform(const float* src, float* dst, int64_t n, bool float3, const Affine& m,
+                      float lo, float hi, hipStream_t s) {
+  if (n <= 0) return;
+  const int blocks = static_cast<int>(std::min<int64_t>((n + 255) / 256, 8192));
+  if (float3)
+    transform_kernel<3><<<blocks, 256, 0, s>>>(src, dst, n, m, lo, hi);
+  else
+    transform_kernel<4><<<blocks, 256, 0, s>>>(src, dst, n, m, lo, hi);
+}
 
 void launch_copy(void* dst, const void* src, int64_t bytes, hipStream_t s) {
   const int64_t n16 = bytes / 16;

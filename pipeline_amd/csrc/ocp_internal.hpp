@@ -94,6 +94,11 @@ bool morton_config_supported(int ppt);
 // ---- utility kernels (k_util.hip)
 // bytes must be a multiple of 16 and both pointers 16-byte aligned.
 void launch_copy(void* dst, const void* src, int64_t bytes, hipStream_t s);
+struct Affine {
+  float m[12];  // rows of A (3x3), then t
+};
+void launch_transform(const float* src, float* dst, int64_t n, bool float3, const Affine& m,
+                      float lo, float hi, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t value, int64_t n_words,
                      hipStream_t s);
 // Zero tickets, bump epoch; optionally zero n_hist words of histogram and

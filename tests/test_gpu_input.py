@@ -114,3 +114,59 @@ def test_device_boxes_512_clouds(ctx):
         np.testing.assert_array_equal(k_dev, k_host)
     finally:
         d.free()
+
+
+def _rotation(seed):
+    rng = np.random.default_rng(seed)
+    axis = rng.normal(size=3)
+    axis /= np.linalg.norm(axis)
+    ang = rng.uniform(-0.5, 0.5)
+    k = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    return np.eye(3) + np.sin(ang) * k + (1 - np.cos(ang)) * (k @ k)
+
+
+@pytest.mark.parametrize("float4,n,in_place", [(True, 3_000_001, False), (False, 3_000_001, True),
+                                               (True, 1000, True), (False, 0, False)])
+def test_transform_points(ctx, float4, n, in_place):
+    """ocp_transform_points against float64 A p + t (rounded once to float32
+    the result differs by at most a few ulp of 1024), with clipping exact at
+    both bounds and the fourth value of float4 points passed through bit for
+    bit; the identity copies every point exactly."""
+    pts = clouds.uniform(n, seed=3)
+    host = pts if float4 else np.ascontiguousarray(pts[:, :3])
+    rot = _rotation(11)
+    c = np.array([512.0, 512.0, 512.0])
+    aff = np.hstack([rot, (c + np.array([3.0, -40.0, 0.25]) - rot @ c)[:, None]])
+    hi = float(np.nextafter(np.float32(1024.0), np.float32(0.0)))
+    src = ctx.upload(host)
+    dst = src if in_place else ctx.empty(host.shape, np.float32)
+    try:
+        ctx.transform_points(src, dst, aff, 0.0, hi)
+        got = dst.to_host().reshape(host.shape)
+        exp = np.clip(host[:, :3].astype(np.float64) @ rot.T + aff[:, 3], 0.0, hi)
+        np.testing.assert_allclose(got[:, :3], exp, rtol=0, atol=1e-3)
+        clipped_lo, clipped_hi = exp <= 0.0, exp >= hi
+        assert np.all(got[:, :3][clipped_lo] == 0.0) and np.all(got[:, :3][clipped_hi] == np.float32(hi))
+        if n >= 1000:
+            assert clipped_lo.any()  # the -40 shift in y pushes points out at 0
+        if float4:
+            np.testing.assert_array_equal(got[:, 3].view(np.uint32), host[:, 3].view(np.uint32))
+        back = ctx.upload(host)
+        ctx.transform_points(back, back, np.hstack([np.eye(3), np.zeros((3, 1))]), 0.0, hi)
+        np.testing.assert_array_equal(back.to_host().reshape(host.shape).view(np.uint32),
+                                      host.view(np.uint32))
+        back.free()
+    finally:
+        src.free()
+        if not in_place:
+            dst.free()
+
+
+def test_transform_points_rejects_bad_arguments(ctx):
+    from pipeline_amd import OcpError
+    d = ctx.empty((10, 4), np.float32)
+    try:
+        with pytest.raises(OcpError):
+            ctx.transform_points(d, d, np.zeros((3, 4)), 2.0, 1.0)
+    finally:
+        d.free()
