@@ -79,6 +79,9 @@ int ocp_free(ocp_ctx ctx, void* ptr);
 int ocp_memcpy_h2d(ocp_ctx ctx, void* dst, const void* src, size_t bytes);
 int ocp_memcpy_d2h(ocp_ctx ctx, void* dst, const void* src, size_t bytes);
 int ocp_memset(ocp_ctx ctx, void* dst, int value, size_t bytes);
+/* Free and total bytes of the context's device (hipMemGetInfo), e.g. to
+ * size a batch against ocp_workspace_size. */
+int ocp_device_memory(ocp_ctx ctx, int64_t* free_bytes, int64_t* total_bytes);
 /* Device-to-device copy kernel on the context stream (16-byte aligned
  * pointers, size a multiple of 16). */
 int ocp_copy(ocp_ctx ctx, void* dst, const void* src, size_t bytes);

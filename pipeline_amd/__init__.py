@@ -122,6 +122,7 @@ SIGNATURES = {
     "ocp_synchronize": (_INT, [_P]),
     "ocp_set_wait_limit": (_INT, [_P, _I64]),
     "ocp_malloc": (_INT, [_P, ctypes.POINTER(_P), _SZ]),
+    "ocp_device_memory": (_INT, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "ocp_free": (_INT, [_P, _P]),
     "ocp_memcpy_h2d": (_INT, [_P, _P, _P, _SZ]),
     "ocp_memcpy_d2h": (_INT, [_P, _P, _P, _SZ]),
@@ -338,6 +339,12 @@ class Context:
     def upload(self, host: np.ndarray, dtype=None) -> DeviceArray:
         h = np.ascontiguousarray(host, dtype=dtype or host.dtype)
         return self.empty(h.shape, h.dtype).copy_from(h)
+
+    def device_memory(self):
+        """(free, total) bytes of the context's device."""
+        fr, tot = _I64(), _I64()
+        self._check(self.lib.ocp_device_memory(self.h, ctypes.byref(fr), ctypes.byref(tot)))
+        return int(fr.value), int(tot.value)
 
     def synchronize(self):
         self._check(self.lib.ocp_synchronize(self.h))

@@ -805,6 +805,15 @@ int ocp_malloc(ocp_ctx c, void** ptr, size_t bytes) {
   OCP_HIP(c, hipMalloc(ptr, bytes ? bytes : 1));
   return OCP_OK;
 }
+int ocp_device_memory(ocp_ctx c, int64_t* free_bytes, int64_t* total_bytes) {
+  if (check_ctx(c) || !free_bytes || !total_bytes) return OCP_ERR_INVALID;
+  OCP_HIP(c, hipSetDevice(c->device));
+  size_t fr = 0, tot = 0;
+  OCP_HIP(c, hipMemGetInfo(&fr, &tot));
+  *free_bytes = static_cast<int64_t>(fr);
+  *total_bytes = static_cast<int64_t>(tot);
+  return OCP_OK;
+}
 int ocp_free(ocp_ctx c, void* ptr) {
   if (check_ctx(c)) return OCP_ERR_INVALID;
   OCP_HIP(c, hipStreamSynchronize(c->stream));

@@ -290,7 +290,6 @@ def test_autotune_cache(ctx, tmp_path):
 def test_workspace_size_matches_device_memory():
     """ocp_workspace_size predicts the device memory a fresh context takes
     for its first build (measured as the drop in free device memory)."""
-    torch = pytest.importorskip("torch")
     from pipeline_amd import Context, workspace_size
     n = 10_000_000
     pts = clouds.uniform(n, seed=5)
@@ -298,10 +297,10 @@ def test_workspace_size_matches_device_memory():
     try:
         d = c.upload(pts)
         c.synchronize()
-        free0, _ = torch.cuda.mem_get_info()
+        free0, _ = c.device_memory()
         c.build(d, n, 0.0, 1024.0)
         c.synchronize()
-        free1, _ = torch.cuda.mem_get_info()
+        free1, _ = c.device_memory()
         used = free0 - free1
         want = workspace_size(n)["build_bytes"]
         assert abs(used - want) <= 64 * 2**20 + 0.02 * want, (used, want)
