@@ -365,7 +365,8 @@ def bench_learned(ctx, clouds, frames: int, n: int = 1 << 26, reps: int = 5):
         exact = bool(np.array_equal(work.to_host(), ref.to_host()))
         per.append({"frame": t, "onesweep_ms": round(ms_os, 4), "learned_ms": round(ms_l, 4),
                     "exact": exact, "train_ms": round(train_ms, 1), "loss": loss,
-                    "keys_fallback": st["keys_big"], "max_bucket": st["max_bucket"]})
+                    "keys_fallback": st["keys_big"], "max_bucket": st["max_bucket"],
+                    "fallback_passes": st["fallback_passes"]})
         keys_prev, keys_cur = keys_cur, keys_prev
     ctx.cdf_reset(0, (1 << 30) - 1)
     ms_lin = timed(lambda: ctx.sort_learned(work, alt, n), keys_prev)

@@ -470,7 +470,8 @@ typedef struct ocp_learned_stats {
   int32_t misordered;         /* bucket boundaries found out of order */
   int64_t keys_big;           /* keys in the fallback buckets */
   int32_t full_resort;        /* 1 if the whole array was re-sorted by OneSweep */
-  int32_t reserved;
+  int32_t fallback_passes;    /* digit passes the fallback ran (0, 2 or 4): those
+                                 whose digit varies inside some oversized bucket */
 } ocp_learned_stats;
 
 /* The linear CDF over [kmin, kmax] (identity root, uniform leaves). */

@@ -72,10 +72,10 @@ class Tune(ctypes.Structure):
 class LearnedStats(ctypes.Structure):
     _fields_ = [("n", _I64), ("n_buckets_nonempty", _I32), ("max_bucket", _I32),
                 ("n_big", _I32), ("misordered", _I32), ("keys_big", _I64),
-                ("full_resort", _I32), ("reserved", _I32)]
+                ("full_resort", _I32), ("fallback_passes", _I32)]
 
     def as_dict(self):
-        return {f: int(getattr(self, f)) for f, _ in self._fields_ if f != "reserved"}
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
 
 
 class WorkspaceSizes(ctypes.Structure):

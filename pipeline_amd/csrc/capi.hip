@@ -1983,8 +1983,16 @@ int ocp_sort_learned(ocp_ctx c, uint32_t* keys, uint32_t* alt, uint32_t n,
   local.n_big = n_big;
   local.keys_big = static_cast<uint32_t>(L.h_info[kInfoBigKeys]);
   if (n_big > tb.big_cap) return fail(c, OCP_ERR_STATE, "oversized-bucket table overflow");
-  if (n_big > 0) {
-    // Oversized buckets: segmented OneSweep, in place (4 passes end in keys).
+  // Digits above the highest bit in which an oversized bucket's key range
+  // varies are constant in every such bucket: their passes are identities
+  // and are left out. An odd count takes one more (an identity copy-through)
+  // so the keys end in `keys`.
+  const uint32_t vary = static_cast<uint32_t>(L.h_info[kInfoVary]);
+  int fb_passes = vary ? (31 - __builtin_clz(vary)) / 8 + 1 : 0;
+  fb_passes += fb_passes & 1;
+  local.fallback_passes = n_big > 0 ? fb_passes : 0;
+  if (n_big > 0 && fb_passes > 0) {
+    // Oversized buckets: segmented OneSweep, in place.
     launch_fill_u32(tb.big_hist, 0u, int64_t(n_big) * 1024, c->stream);
     SegView sv{};
     sv.n_seg = n_big;
@@ -1995,7 +2003,7 @@ int ocp_sort_learned(ocp_ctx c, uint32_t* keys, uint32_t* alt, uint32_t n,
                            c->stream);
     uint32_t* src = keys;
     uint32_t* dst = alt;
-    for (int pass = 0; pass < 4; ++pass) {
+    for (int pass = 0; pass < fb_passes; ++pass) {
       launch_onesweep_pass(src, dst, sv, tb.big_tiles, big_tiles, tb.big_hist, c->s_flags,
                            c->ctrl, pass, pass != 0, c->tune, c->stream);
       std::swap(src, dst);

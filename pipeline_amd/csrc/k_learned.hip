@@ -253,11 +253,21 @@ __global__ __launch_bounds__(1024) void lprefix_kernel(LearnTables tb, int32_t f
   __shared__ uint32_t tmp[16];
   __shared__ int32_t s_bs[kLearnBuckets + 1];
   __shared__ int32_t s_max;
+  __shared__ uint32_t s_vary;
   constexpr int PER = kLearnBuckets / 1024;
   const int tid = threadIdx.x;
-  if (tid == 0) s_max = 0;
+  if (tid == 0) {
+    s_max = 0;
+    s_vary = 0;
+  }
   uint32_t v[PER];
   uint32_t sum = 0, nonempty = 0, nbig = 0, big_keys = 0, big_tiles = 0;
+  // Keys land in bucket b exactly when fb[b] <= key < fb[b+1] (bucket 0
+  // takes every key below fb[1], the last one every key from fb[16383] on:
+  // the search returns the largest f with fb[f] <= key), so the bits above
+  // the highest bit of lo ^ (hi - 1) are the same for every key of the
+  // bucket and the fallback's passes over those digits are identities.
+  uint32_t vary = 0;
   int32_t mx = 0;
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
@@ -266,6 +276,10 @@ __global__ __launch_bounds__(1024) void lprefix_kernel(LearnTables tb, int32_t f
     nonempty += v[q] ? 1u : 0u;
     mx = max(mx, static_cast<int32_t>(v[q]));
     if (v[q] > static_cast<uint32_t>(kLearnBigCap)) {
+      const int32_t b = tid * PER + q;
+      const uint32_t lo = b == 0 ? 0u : tb.fb[b];
+      const uint32_t hi = b + 1 < kLearnBuckets ? tb.fb[b + 1] : 0u;
+      vary |= (b + 1 < kLearnBuckets && hi > lo) ? (lo ^ (hi - 1u)) : 0xffffffffu;
       ++nbig;
       big_keys += v[q];
       big_tiles += (v[q] + fallback_tile - 1) / fallback_tile;
@@ -283,7 +297,9 @@ __global__ __launch_bounds__(1024) void lprefix_kernel(LearnTables tb, int32_t f
     s_bs[kLearnBuckets] = static_cast<int32_t>(total);
     tb.bstart[kLearnBuckets] = static_cast<int32_t>(total);
   }
+  // (after the scan's barriers: s_max and s_vary are cleared before them)
   atomicMax(&s_max, mx);
+  if (vary) atomicOr(&s_vary, vary);
   // oversized buckets, in bucket order
   uint32_t n_big_all, n_tiles_all, keys_all, ne_all;
   uint32_t bi = block_excl_scan<1024>(nbig, tmp, &n_big_all);
@@ -328,6 +344,7 @@ __global__ __launch_bounds__(1024) void lprefix_kernel(LearnTables tb, int32_t f
     tb.info[kInfoBigKeys] = static_cast<int32_t>(keys_all);
     tb.info[kInfoMaxBucket] = s_max;
     tb.info[kInfoNonEmpty] = static_cast<int32_t>(ne_all);
+    tb.info[kInfoVary] = static_cast<int32_t>(s_vary);
   }
 }
 

@@ -62,6 +62,7 @@ enum LearnInfo : int {
   kInfoBigKeys = 2,     // keys in them
   kInfoMaxBucket = 3,   // largest bucket
   kInfoNonEmpty = 4,    // non-empty buckets
+  kInfoVary = 5,        // OR of fb[q] ^ (fb[q+1] - 1) over oversized buckets q
 };
 
 // ---- sort

@@ -184,6 +184,37 @@ def test_gpu_sort_learned_bad_models_fall_back(ctx):
 
 
 @pytest.mark.gpu
+def test_gpu_fallback_sorts_only_varying_digits(ctx):
+    """The OneSweep fallback runs only the digit passes in which some
+    oversized bucket's key range [fb[q], fb[q+1]) varies (rounded up to an
+    even count, so the keys end in place): recomputed here from the model's
+    own boundaries, and the output still equals np.sort."""
+    ctx.cdf_reset(0, KMAX30)
+    fb = ctx.cdf_boundaries().astype(np.int64)
+    rng = np.random.default_rng(7)
+    # bucket 1 of the linear CDF is exactly [2^16, 2^17): two digits vary;
+    # bucket 10000 straddles a 2^20 boundary: three vary, so four passes
+    for name, lo, span, want in (("aligned bucket", int(fb[1]) + 100, 4000, 2),
+                                 ("straddling bucket", int(fb[10000]) + 10, 60000, 4)):
+        h = (lo + rng.integers(0, span, 50_000)).astype(np.uint32)
+        counts = np.bincount(cdf.bucket_of(fb.astype(np.uint32), h), minlength=fb.size)
+        vary = 0
+        for q in np.nonzero(counts > 16384)[0]:
+            q_lo = 0 if q == 0 else int(fb[q])
+            q_hi = int(fb[q + 1]) if q + 1 < fb.size else 0
+            vary |= (q_lo ^ (q_hi - 1)) if (q + 1 < fb.size and q_hi > q_lo) else 0xFFFFFFFF
+        need = (vary.bit_length() + 7) // 8
+        assert need + (need & 1) == want, (name, hex(vary))
+        d = ctx.upload(h)
+        alt = ctx.empty(h.size, np.uint32)
+        st = ctx.sort_learned(d, alt, h.size)
+        np.testing.assert_array_equal(d.to_host(), np.sort(h), err_msg=name)
+        assert st["n_big"] == 1 and st["fallback_passes"] == want, (name, st)
+        d.free()
+        alt.free()
+
+
+@pytest.mark.gpu
 def test_gpu_trained_model_beats_linear_cdf(ctx):
     """What the trained network gains over the untrained (linear-CDF)
     bucketing on clustered keys: fewer keys left to the fallback sort and a
