@@ -470,8 +470,9 @@ typedef struct ocp_learned_stats {
   int32_t misordered;         /* bucket boundaries found out of order */
   int64_t keys_big;           /* keys in the fallback buckets */
   int32_t full_resort;        /* 1 if the whole array was re-sorted by OneSweep */
-  int32_t fallback_passes;    /* digit passes the fallback ran (0, 2 or 4): those
-                                 whose digit varies inside some oversized bucket */
+  int32_t fallback_passes;    /* digit passes the fallback ran: 2 over the oversized
+                                 buckets whose key range varies in the low two
+                                 digits only, plus 4 over the others (0, 2, 4, 6) */
 } ocp_learned_stats;
 
 /* The linear CDF over [kmin, kmax] (identity root, uniform leaves). */

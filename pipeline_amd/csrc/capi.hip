@@ -1733,7 +1733,7 @@ static int learn_tables_alloc(ocp_ctx c, int64_t n) {
     dev_free(t.big_start); dev_free(t.big_end); dev_free(t.big_tiles); dev_free(t.big_hist);
     int r = 0;
     if ((r = dev_alloc(c, &t.big_start, big_cap)) || (r = dev_alloc(c, &t.big_end, big_cap)) ||
-        (r = dev_alloc(c, &t.big_tiles, int64_t(big_cap) + 1)) ||
+        (r = dev_alloc(c, &t.big_tiles, int64_t(big_cap) + 2)) ||
         (r = dev_alloc(c, &t.big_hist, int64_t(big_cap) * 1024)))
       return r;
     t.big_cap = big_cap;
@@ -1983,32 +1983,39 @@ int ocp_sort_learned(ocp_ctx c, uint32_t* keys, uint32_t* alt, uint32_t n,
   local.n_big = n_big;
   local.keys_big = static_cast<uint32_t>(L.h_info[kInfoBigKeys]);
   if (n_big > tb.big_cap) return fail(c, OCP_ERR_STATE, "oversized-bucket table overflow");
-  // Digits above the highest bit in which an oversized bucket's key range
-  // varies are constant in every such bucket: their passes are identities
-  // and are left out. An odd count takes one more (an identity copy-through)
-  // so the keys end in `keys`.
-  const uint32_t vary = static_cast<uint32_t>(L.h_info[kInfoVary]);
-  int fb_passes = vary ? (31 - __builtin_clz(vary)) / 8 + 1 : 0;
-  fb_passes += fb_passes & 1;
-  local.fallback_passes = n_big > 0 ? fb_passes : 0;
-  if (n_big > 0 && fb_passes > 0) {
-    // Oversized buckets: segmented OneSweep, in place.
-    launch_fill_u32(tb.big_hist, 0u, int64_t(n_big) * 1024, c->stream);
+  // Oversized buckets: segmented OneSweep, in place. Keys reach bucket q
+  // only from its range [fb[q], fb[q+1]), so the digits above that range's
+  // highest varying bit are constant in the bucket and their passes are
+  // identities: buckets whose range varies in the low two digits only
+  // (group A) take two passes, the others (group B) four; both counts are
+  // even, so the keys end in `keys`. Group B's passes reuse the pass stage
+  // ids, so they run in a new epoch (flags of group A's passes then carry a
+  // stale tag).
+  const int32_t n_a = std::min(L.h_info[kInfoBigA], n_big);
+  const int32_t tiles_a = L.h_info[kInfoTilesA];
+  auto fallback = [&](int32_t seg0, int32_t n_seg, int32_t* tile_map, int32_t tiles,
+                      int passes, bool new_epoch) {
+    if (n_seg <= 0) return;
+    if (new_epoch) launch_begin(c->ctrl, nullptr, 0, c->stream);
+    uint32_t* hist = tb.big_hist + int64_t(seg0) * 1024;
+    launch_fill_u32(hist, 0u, int64_t(n_seg) * 1024, c->stream);
     SegView sv{};
-    sv.n_seg = n_big;
-    sv.start = tb.big_start;
-    sv.end = tb.big_end;
+    sv.n_seg = n_seg;
+    sv.start = tb.big_start + seg0;
+    sv.end = tb.big_end + seg0;
     sv.n1 = ni;
-    launch_histogram_tiles(keys, sv, tb.big_tiles, big_tiles, stile, tb.big_hist, c->tune,
-                           c->stream);
+    launch_histogram_tiles(keys, sv, tile_map, tiles, stile, hist, c->tune, c->stream);
     uint32_t* src = keys;
     uint32_t* dst = alt;
-    for (int pass = 0; pass < fb_passes; ++pass) {
-      launch_onesweep_pass(src, dst, sv, tb.big_tiles, big_tiles, tb.big_hist, c->s_flags,
-                           c->ctrl, pass, pass != 0, c->tune, c->stream);
+    for (int pass = 0; pass < passes; ++pass) {
+      launch_onesweep_pass(src, dst, sv, tile_map, tiles, hist, c->s_flags, c->ctrl, pass,
+                           pass != 0, c->tune, c->stream);
       std::swap(src, dst);
     }
-  }
+    local.fallback_passes += passes;
+  };
+  fallback(0, n_a, tb.big_tiles, tiles_a, 2, false);
+  fallback(n_a, n_big - n_a, tb.big_tiles + n_a + 1, big_tiles - tiles_a, 4, n_a > 0);
   launch_lverify(keys, tb, ni, c->ctrl, c->stream);
   OCP_CHECK_LAUNCH(c);
   uint32_t cw[kCtrlWords];

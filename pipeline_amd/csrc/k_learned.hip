@@ -261,13 +261,16 @@ __global__ __launch_bounds__(1024) void lprefix_kernel(LearnTables tb, int32_t f
     s_vary = 0;
   }
   uint32_t v[PER];
-  uint32_t sum = 0, nonempty = 0, nbig = 0, big_keys = 0, big_tiles = 0;
+  uint32_t sum = 0, nonempty = 0, big_keys = 0;
   // Keys land in bucket b exactly when fb[b] <= key < fb[b+1] (bucket 0
   // takes every key below fb[1], the last one every key from fb[16383] on:
   // the search returns the largest f with fb[f] <= key), so the bits above
   // the highest bit of lo ^ (hi - 1) are the same for every key of the
   // bucket and the fallback's passes over those digits are identities.
-  uint32_t vary = 0;
+  // Oversized buckets whose range varies in the low two digits only form
+  // group A (two passes), the others group B (four passes).
+  uint32_t vary = 0, wide = 0;  // wide: bit q set for a group-B bucket
+  uint32_t nbig_a = 0, nbig_b = 0, tiles_a = 0, tiles_b = 0;
   int32_t mx = 0;
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
@@ -279,10 +282,19 @@ __global__ __launch_bounds__(1024) void lprefix_kernel(LearnTables tb, int32_t f
       const int32_t b = tid * PER + q;
       const uint32_t lo = b == 0 ? 0u : tb.fb[b];
       const uint32_t hi = b + 1 < kLearnBuckets ? tb.fb[b + 1] : 0u;
-      vary |= (b + 1 < kLearnBuckets && hi > lo) ? (lo ^ (hi - 1u)) : 0xffffffffu;
-      ++nbig;
+      const uint32_t span =
+          (b + 1 < kLearnBuckets && hi > lo) ? (lo ^ (hi - 1u)) : 0xffffffffu;
+      vary |= span;
+      const uint32_t tiles = (v[q] + fallback_tile - 1) / fallback_tile;
       big_keys += v[q];
-      big_tiles += (v[q] + fallback_tile - 1) / fallback_tile;
+      if (span >> 16) {
+        wide |= 1u << q;
+        ++nbig_b;
+        tiles_b += tiles;
+      } else {
+        ++nbig_a;
+        tiles_a += tiles;
+      }
     }
   }
   uint32_t total;
@@ -300,25 +312,47 @@ __global__ __launch_bounds__(1024) void lprefix_kernel(LearnTables tb, int32_t f
   // (after the scan's barriers: s_max and s_vary are cleared before them)
   atomicMax(&s_max, mx);
   if (vary) atomicOr(&s_vary, vary);
-  // oversized buckets, in bucket order
-  uint32_t n_big_all, n_tiles_all, keys_all, ne_all;
-  uint32_t bi = block_excl_scan<1024>(nbig, tmp, &n_big_all);
-  uint32_t bt = block_excl_scan<1024>(big_tiles, tmp, &n_tiles_all);
+  // Oversized buckets, in bucket order within each group: group A takes
+  // segments [0, nA) and tile map big_tiles[0 .. nA], group B segments
+  // [nA, nA + nB) and tile map big_tiles[nA + 1 .. nA + nB + 1] (each map
+  // starts at 0). Nothing is written past big_cap segments or big_cap + 2
+  // map entries; the host refuses more than big_cap oversized buckets.
+  uint32_t na_all, nb_all, ta_all, tb_all, keys_all, ne_all;
+  uint32_t ia = block_excl_scan<1024>(nbig_a, tmp, &na_all);
+  uint32_t ib = block_excl_scan<1024>(nbig_b, tmp, &nb_all);
+  uint32_t ta = block_excl_scan<1024>(tiles_a, tmp, &ta_all);
+  uint32_t tbb = block_excl_scan<1024>(tiles_b, tmp, &tb_all);
   (void)block_excl_scan<1024>(big_keys, tmp, &keys_all);
   (void)block_excl_scan<1024>(nonempty, tmp, &ne_all);
+  const uint32_t cap = static_cast<uint32_t>(tb.big_cap);
+  const uint32_t map_b = na_all + 1u;  // first map entry of group B
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
-    if (v[q] > static_cast<uint32_t>(kLearnBigCap) && static_cast<int32_t>(bi) < tb.big_cap) {
+    if (v[q] > static_cast<uint32_t>(kLearnBigCap)) {
       const int32_t b = tid * PER + q;
-      tb.big_start[bi] = s_bs[b];
-      tb.big_end[bi] = s_bs[b + 1];
-      tb.big_tiles[bi] = static_cast<int32_t>(bt);
-      bt += (v[q] + fallback_tile - 1) / fallback_tile;
-      ++bi;
+      const uint32_t tiles = (v[q] + fallback_tile - 1) / fallback_tile;
+      const bool in_b = (wide >> q) & 1u;
+      const uint32_t seg = in_b ? na_all + ib : ia;
+      const uint32_t map = in_b ? map_b + ib : ia;
+      if (seg < cap && map < cap + 2u) {
+        tb.big_start[seg] = s_bs[b];
+        tb.big_end[seg] = s_bs[b + 1];
+        tb.big_tiles[map] = static_cast<int32_t>(in_b ? tbb : ta);
+      }
+      if (in_b) {
+        tbb += tiles;
+        ++ib;
+      } else {
+        ta += tiles;
+        ++ia;
+      }
     }
   }
-  if (tid == 0) tb.big_tiles[min(static_cast<int32_t>(n_big_all), tb.big_cap)] =
-      static_cast<int32_t>(n_tiles_all);
+  const uint32_t n_big_all = na_all + nb_all;
+  if (tid == 0) {
+    if (na_all < cap + 2u) tb.big_tiles[na_all] = static_cast<int32_t>(ta_all);
+    if (map_b + nb_all < cap + 2u) tb.big_tiles[map_b + nb_all] = static_cast<int32_t>(tb_all);
+  }
   // coarse segments: counts, starts, tile map; fine counts per segment
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
@@ -340,7 +374,9 @@ __global__ __launch_bounds__(1024) void lprefix_kernel(LearnTables tb, int32_t f
     tb.tile_start[kLearnCoarse] = static_cast<int32_t>(ttot);
     *tb.tiles_b = static_cast<int32_t>(ttot);
     tb.info[kInfoBig] = static_cast<int32_t>(n_big_all);
-    tb.info[kInfoBigTiles] = static_cast<int32_t>(n_tiles_all);
+    tb.info[kInfoBigTiles] = static_cast<int32_t>(ta_all + tb_all);
+    tb.info[kInfoBigA] = static_cast<int32_t>(na_all);
+    tb.info[kInfoTilesA] = static_cast<int32_t>(ta_all);
     tb.info[kInfoBigKeys] = static_cast<int32_t>(keys_all);
     tb.info[kInfoMaxBucket] = s_max;
     tb.info[kInfoNonEmpty] = static_cast<int32_t>(ne_all);

@@ -51,7 +51,7 @@ struct LearnTables {
   int32_t* tiles_b;     // [1]
   int32_t* big_start;   // [big_cap] oversized buckets, as segments
   int32_t* big_end;     // [big_cap]
-  int32_t* big_tiles;   // [big_cap + 1] their tile map (fallback sort tile)
+  int32_t* big_tiles;   // [big_cap + 2] their tile maps, group A's then group B's
   int32_t* info;        // [8] see LearnInfo
   uint32_t* big_hist;   // [big_cap * 1024] digit histograms of the oversized buckets
   int32_t big_cap;
@@ -63,6 +63,8 @@ enum LearnInfo : int {
   kInfoMaxBucket = 3,   // largest bucket
   kInfoNonEmpty = 4,    // non-empty buckets
   kInfoVary = 5,        // OR of fb[q] ^ (fb[q+1] - 1) over oversized buckets q
+  kInfoBigA = 6,        // oversized buckets spanning two digits or fewer (group A)
+  kInfoTilesA = 7,      // group A's fallback tiles
 };
 
 // ---- sort

@@ -185,31 +185,34 @@ def test_gpu_sort_learned_bad_models_fall_back(ctx):
 
 @pytest.mark.gpu
 def test_gpu_fallback_sorts_only_varying_digits(ctx):
-    """The OneSweep fallback runs only the digit passes in which some
-    oversized bucket's key range [fb[q], fb[q+1]) varies (rounded up to an
-    even count, so the keys end in place): recomputed here from the model's
-    own boundaries, and the output still equals np.sort."""
+    """The OneSweep fallback sorts each oversized bucket only over the digits
+    its key range [fb[q], fb[q+1]) can vary in: two passes for the buckets
+    whose range varies in the low two digits only, four for the others, each
+    group in place. The count is recomputed here from the model's own
+    boundaries, and the output still equals np.sort."""
     ctx.cdf_reset(0, KMAX30)
     fb = ctx.cdf_boundaries().astype(np.int64)
     rng = np.random.default_rng(7)
-    # bucket 1 of the linear CDF is exactly [2^16, 2^17): two digits vary;
-    # bucket 10000 straddles a 2^20 boundary: three vary, so four passes
-    for name, lo, span, want in (("aligned bucket", int(fb[1]) + 100, 4000, 2),
-                                 ("straddling bucket", int(fb[10000]) + 10, 60000, 4)):
-        h = (lo + rng.integers(0, span, 50_000)).astype(np.uint32)
+    aligned = (int(fb[1]) + 100, 4000)         # bucket 1 is exactly [2^16, 2^17)
+    straddling = (int(fb[10000]) + 10, 60000)  # bucket 10000 spans a 2^20 boundary
+    for name, ranges, want in (("aligned bucket", [aligned], 2),
+                               ("straddling bucket", [straddling], 4),
+                               ("one of each", [aligned, straddling], 6)):
+        h = np.concatenate([lo + rng.integers(0, span, 50_000) for lo, span in ranges])
+        h = rng.permutation(h).astype(np.uint32)
         counts = np.bincount(cdf.bucket_of(fb.astype(np.uint32), h), minlength=fb.size)
-        vary = 0
+        groups = set()
         for q in np.nonzero(counts > 16384)[0]:
             q_lo = 0 if q == 0 else int(fb[q])
             q_hi = int(fb[q + 1]) if q + 1 < fb.size else 0
-            vary |= (q_lo ^ (q_hi - 1)) if (q + 1 < fb.size and q_hi > q_lo) else 0xFFFFFFFF
-        need = (vary.bit_length() + 7) // 8
-        assert need + (need & 1) == want, (name, hex(vary))
+            span = (q_lo ^ (q_hi - 1)) if (q + 1 < fb.size and q_hi > q_lo) else 0xFFFFFFFF
+            groups.add(4 if span >> 16 else 2)
+        assert sum(groups) == want, (name, groups)
         d = ctx.upload(h)
         alt = ctx.empty(h.size, np.uint32)
         st = ctx.sort_learned(d, alt, h.size)
         np.testing.assert_array_equal(d.to_host(), np.sort(h), err_msg=name)
-        assert st["n_big"] == 1 and st["fallback_passes"] == want, (name, st)
+        assert st["n_big"] == len(ranges) and st["fallback_passes"] == want, (name, st)
         d.free()
         alt.free()
 
