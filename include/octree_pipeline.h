@@ -257,7 +257,8 @@ int ocp_workspace_size(const struct ocp_tune* tune, int64_t n, int32_t n_seg,
 int ocp_build_autobox(ocp_ctx ctx, const float* points, int float3, int64_t n);
 /* Moves a cloud in device memory: p' = clamp(A p + t, lo, hi) per coordinate,
  * with affine = {A row-major (9 floats), t (3 floats)}; a float4 point's
- * fourth value passes through; src == dst is allowed. Successive frames of a
+ * fourth value passes through; src and dst are the same buffer or do not
+ * overlap. Asynchronous on the context stream. Successive frames of a
  * moving sensor are made this way (rigid motion: A a rotation). No
  * counterpart in the reference, whose frames are files on disk. */
 int ocp_transform_points(ocp_ctx ctx, const float* src, float* dst, int float3, int64_t n,

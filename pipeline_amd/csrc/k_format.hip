@@ -312,7 +312,8 @@ __global__ __launch_bounds__(kFmtThreads) void expand_kernel(
     int64_t o = cn.first_child;
     for (int q = 0; q < 8; ++q) {
       if (!((cn.child_set >> q) & 1u)) continue;
-      nd.children[q] = o < n_children ? children[o] : 0;
+      // a malformed node (offset outside [0, n_children)) reads no memory
+      nd.children[q] = (o >= 0 && o < n_children) ? children[o] : 0;
       ++o;
     }
     if (cn.level != 0xff) {
