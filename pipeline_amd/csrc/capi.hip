@@ -1579,42 +1579,51 @@ int ocp_autotune(ocp_ctx c, const float* points, int float3, int64_t n,
   ocp_get_tune(c, &best);
   float best_ms = 0;
   if (int r = time_cfg(best, &best_ms)) return r;
-  // Coordinate descent: sort tile of 8-bit digits, then the pass count
-  // (four 8-bit passes or three 10-bit ones, two tile sizes), then scan
-  // tile, then Morton tile/grid.
+  // Coordinate descent, two rounds: sort tile of 8-bit digits, then the
+  // pass count (four 8-bit passes or three 10-bit ones, two tile sizes),
+  // then scan tile, then Morton tile/grid. The second round revisits the
+  // pass count under the chosen Morton shape: the Morton kernel's histogram
+  // flush grows with its grid and three times over with 10-bit digits, so
+  // the two choices interact.
   const int sort_cfgs[][2] = {{8, 15}, {4, 16}, {4, 24}, {8, 8}, {8, 12}, {8, 16}, {16, 8}};
-  for (const auto& sc : sort_cfgs) {
-    ocp_tune t = best;
-    t.sort_waves = sc[0];
-    t.sort_kpt = sc[1];
+  auto try_cfg = [&](const ocp_tune& t) -> int {
     float ms;
     if (int r = time_cfg(t, &ms)) return r;
-    if (ms < best_ms) { best_ms = ms; best = t; }
-  }
-  for (int kpt : {8, 16}) {
-    ocp_tune t = best;
-    t.sort_bits = 10;
-    t.sort10_kpt = kpt;
-    float ms;
-    if (int r = time_cfg(t, &ms)) return r;
-    if (ms < best_ms) { best_ms = ms; best = t; }
-  }
-  for (int ipt : {8, 16, 32}) {
-    ocp_tune t = best;
-    t.scan_ipt = ipt;
-    float ms;
-    if (int r = time_cfg(t, &ms)) return r;
-    if (ms < best_ms) { best_ms = ms; best = t; }
-  }
-  for (int ppt : {4, 8, 16, 32}) {
-    for (int grid : {512, 1024, 2048}) {
-      ocp_tune t = best;
-      t.morton_ppt = ppt;
-      t.morton_grid = grid;
-      float ms;
-      if (int r = time_cfg(t, &ms)) return r;
-      if (ms < best_ms) { best_ms = ms; best = t; }
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = t;
     }
+    return OCP_OK;
+  };
+  for (int round = 0; round < 2; ++round) {
+    const ocp_tune start = best;
+    for (const auto& sc : sort_cfgs) {
+      ocp_tune t = best;
+      t.sort_bits = 8;
+      t.sort_waves = sc[0];
+      t.sort_kpt = sc[1];
+      if (int r = try_cfg(t)) return r;
+    }
+    for (int kpt : {8, 16}) {
+      ocp_tune t = best;
+      t.sort_bits = 10;
+      t.sort10_kpt = kpt;
+      if (int r = try_cfg(t)) return r;
+    }
+    for (int ipt : {8, 16, 32}) {
+      ocp_tune t = best;
+      t.scan_ipt = ipt;
+      if (int r = try_cfg(t)) return r;
+    }
+    for (int ppt : {4, 8, 16, 32}) {
+      for (int grid : {256, 512, 1024, 2048}) {
+        ocp_tune t = best;
+        t.morton_ppt = ppt;
+        t.morton_grid = grid;
+        if (int r = try_cfg(t)) return r;
+      }
+    }
+    if (std::memcmp(&start, &best, sizeof(best)) == 0) break;
   }
   if (int r = ocp_set_tune(c, &best)) return r;
   if (chosen) *chosen = best;
