@@ -525,8 +525,7 @@ bool enqueue_octree(ocp_ctx c, const Plan& p, bool capturing) {
     launch_oct_offsets(kalloc, p.n_seg, w.oct_start, c->ctrl, w.k_cap, s); }
   const OctArgs a = pipeline_oct_args(c);
   KScope k(c, kKOctBuild, capturing);
-  return launch_oct_build(a, p.sv, p.ts_node, p.tiles_node,
-                          kOctMake | kOctLink | kOctRoot | kOctFull, s);
+  return launch_oct_build(a, p.sv, p.ts_node, p.tiles_node, s);
 }
 
 void mark(ocp_ctx c, int k, bool capturing) {

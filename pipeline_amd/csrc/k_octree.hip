@@ -663,9 +663,7 @@ void launch_oct_root(OctNode* nodes, const uint32_t* codes,
 }
 
 bool launch_oct_build(const OctArgs& a, const SegView& sv,
-                      const int32_t* tile_start, int32_t total_tiles, int mode,
-                      hipStream_t s) {
-  (void)mode;  // the fused build always writes every slot
+                      const int32_t* tile_start, int32_t total_tiles, hipStream_t s) {
   // The group walk reads the children arrays; the per-segment counts, node
   // bases and the fit flag come from the build.
   if (!a.has_leaf_left || !a.has_leaf_right || !a.left_child || !a.codes ||

@@ -21,10 +21,7 @@ __global__ __launch_bounds__(256) void begin_kernel(uint32_t* ctrl,
                                                     uint32_t* z2, int64_t n2) {
   const int64_t tid = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (tid == 0) {
-#pragma unroll
-    for (int k = 0; k < 8; ++k) ctrl[kCtrlTicket0 + k] = 0;
     ctrl[kCtrlEpoch] = ctrl[kCtrlEpoch] + 1;
-    ctrl[kCtrlLearnFail] = 0;
     ctrl[kCtrlLearnBad] = 0;
   }
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
@@ -40,8 +37,6 @@ __global__ __launch_bounds__(256) void begin_shared_kernel(uint32_t* ctrl, uint3
                                                            uint32_t* hist, int64_t n_hist) {
   const int64_t tid = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (tid == 0) {
-#pragma unroll
-    for (int k = 0; k < 8; ++k) ctrl[kCtrlTicket0 + k] = 0;
     const uint32_t e = max(ctrl[kCtrlEpoch], *dev_epoch) + 1u;
     ctrl[kCtrlEpoch] = e;
     *dev_epoch = e;

@@ -12,11 +12,11 @@ namespace ocp {
 
 // Control words in device memory (one block per context).
 enum CtrlSlot : int {
-  kCtrlTicket0 = 0,    // 8 tile-ticket counters, one per lookback stage
+  // words 0-7 are unused (tiles take their index from the workgroup id)
   kCtrlEpoch = 8,      // bumped once per launch group; tags lookback flags
   kCtrlOctTotal = 9,   // total octree slots of the batch
   kCtrlOctOk = 10,     // 1 when the octree fits the node capacity
-  kCtrlLearnFail = 11, // learned sort: buckets that overflowed
+  // word 11 is unused
   kCtrlLearnBad = 12,  // learned sort: misordered bucket boundaries
   kCtrlTimeout = 13,   // mark of the launch whose wait gave up (0: none)
   kCtrlWaitLimit = 14, // 0: waits poll up to kSpinLimit times; v > 0: v - 1
@@ -51,7 +51,7 @@ __device__ __forceinline__ bool stopped_any(const uint32_t* ctrl) {
   return ctrl[kCtrlTimeout] != 0u;
 }
 
-// Lookback stage ids (ticket counter and flag tag).
+// Lookback stage ids (part of every flag tag).
 enum Stage : uint32_t {
   kStagePass0 = 0,  // OneSweep passes 0..3
   kStageUnique = 4,
@@ -101,8 +101,8 @@ void launch_transform(const float* src, float* dst, int64_t n, bool float3, cons
                       float lo, float hi, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t value, int64_t n_words,
                      hipStream_t s);
-// Zero tickets, bump epoch; optionally zero n_hist words of histogram and
-// n2 words at z2.
+// Bump the epoch and clear the learned sort's misorder count; optionally
+// zero n_hist words of histogram and n2 words at z2.
 void launch_begin(uint32_t* ctrl, uint32_t* hist, int64_t n_hist,
                   hipStream_t s, uint32_t* z2 = nullptr, int64_t n2 = 0);
 // Begin for lookback flags shared by every context on the device: the new
@@ -213,15 +213,11 @@ struct OctArgs {
 void launch_oct_root(OctNode* nodes, const uint32_t* codes,
                      const uint8_t* prefix_n, int32_t n_brt, float mn,
                      float rg, hipStream_t s);
-// mode bits: 1 = make phase (corners, cell sizes, node links), 2 = leaf
-// links, 4 = root corner and cell size, 8 = every slot including the zero
-// gaps (the fused pipeline passes 1|2|4|8).
-// Needs every OctArgs array (the fused build has them all).
-constexpr int kOctMake = 1, kOctLink = 2, kOctRoot = 4, kOctFull = 8;
-// Returns false (and launches nothing) when an array it needs is null.
+// The fused make + link + root pass: writes every slot, zero gaps included.
+// Needs every OctArgs array (the fused build has them all); returns false
+// (and launches nothing) when an array it needs is null.
 bool launch_oct_build(const OctArgs& a, const SegView& sv,
-                      const int32_t* tile_start, int32_t total_tiles, int mode,
-                      hipStream_t s);
+                      const int32_t* tile_start, int32_t total_tiles, hipStream_t s);
 
 // ---- stage 7, per-stage entry points (k_octree_stage.hip)
 // The make call of the reference signature has no children arrays
