@@ -2000,7 +2000,7 @@ int ocp_sort_learned(ocp_ctx c, uint32_t* keys, uint32_t* alt, uint32_t n,
   // ids, so they run in a new epoch (flags of group A's passes then carry a
   // stale tag).
   const int32_t n_a = std::min(L.h_info[kInfoBigA], n_big);
-  const int32_t tiles_a = L.h_info[kInfoTilesA];
+  const int32_t tiles_a = std::clamp(L.h_info[kInfoTilesA], 0, std::max(big_tiles, 0));
   auto fallback = [&](int32_t seg0, int32_t n_seg, int32_t* tile_map, int32_t tiles,
                       int passes, bool new_epoch) {
     if (n_seg <= 0) return;
