@@ -415,8 +415,11 @@ __device__ __forceinline__ void build_chain(const View& v, int32_t i, int32_t ro
       found_next = true;
     }
     if (pq > last) return true;
-    const int32_t hi = min(pq + eq - 1, last);
-    for (int32_t s = pq; s <= hi; ++s) owned &= ~(1u << (s - p_i));
+    // slots max(pq, p_i) .. min(pq + eq - 1, last) pass to chain q: at most
+    // kMaxEdge bits of `owned`
+    const int32_t lo_s = max(pq, p_i);
+    const int32_t cnt = min(pq + eq - 1, last) - lo_s + 1;
+    if (cnt > 0) owned &= ~(((1u << cnt) - 1u) << (lo_s - p_i));
     return false;
   };
   {
