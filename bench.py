@@ -435,7 +435,7 @@ def main() -> int:
         b_buf.free()
     tuned = None
     if not args.no_autotune:
-        tuned = ctx.autotune(d, n=n, reps=3, cache=args.tune_cache)
+        tuned = ctx.autotune(d, n=n, reps=10, cache=args.tune_cache)
     # Eager builds with per-kernel HIP events: back to back (warm), then with
     # 512 MiB written before every kernel (cold: L2 and the 256 MB Infinity
     # Cache hold none of its input).

@@ -427,8 +427,10 @@ typedef struct ocp_tune {
 } ocp_tune;
 int ocp_get_tune(ocp_ctx ctx, ocp_tune* out);
 int ocp_set_tune(ocp_ctx ctx, const ocp_tune* in);
-/* Time every supported configuration on the given workload (plain when
- * n_seg == 0, else batched) and keep the fastest; writes the choice. */
+/* Time the supported configurations on the given workload (plain when
+ * n_seg == 0, else batched) by two rounds of coordinate descent, each
+ * configuration's time the best of three runs of `reps` builds, and keep
+ * the fastest; writes the choice. */
 int ocp_autotune(ocp_ctx ctx, const float* points, int float3, int64_t n,
                  const int32_t* seg_offsets, const float* seg_min,
                  const float* seg_range, int32_t n_seg, int reps,
