@@ -132,7 +132,7 @@ def cpu_baseline(pts: np.ndarray, c1_pts: np.ndarray, budget_s: float = 20.0):
     nt = omp_threads()
     n = pts.shape[0]
     ms, u, k, reps = _cpu_run(pts, nt, budget_s)
-    ms1, u1, k1, reps1 = _cpu_run(pts, 1, 4.0)   # ~0.5 s a run: five runs
+    ms1, u1, k1, reps1 = _cpu_run(pts, 1, 5.0)   # ~0.5 s a run: five runs
     c_all, _, _, c_reps = _cpu_run(c1_pts, nt, 2.0)
     c_one, _, _, c_reps1 = _cpu_run(c1_pts, 1, 2.0)
     m = c1_pts.shape[0]
