@@ -145,3 +145,44 @@ def test_workspace_size_query():
     assert a["sort_bytes"] <= b["sort_bytes"] and b["scan_bytes"] > 0
     with pytest.raises(pipeline_amd.OcpError):
         pipeline_amd.workspace_size(-1)
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes structures and numpy dtypes of pipeline_amd mirror the
+    header's structs: sizes and every field's offset, as a C compiler lays
+    them out (a silent mismatch would misread every field after it)."""
+    mirrors = {
+        "ocp_results": pipeline_amd.Results,
+        "ocp_tune": pipeline_amd.Tune,
+        "ocp_learned_stats": pipeline_amd.LearnedStats,
+        "ocp_workspace_sizes": pipeline_amd.WorkspaceSizes,
+        "ocp_tree_report": pipeline_amd.TreeReport,
+        "ocp_oct_node": pipeline_amd.OCT_NODE_DTYPE,
+        "ocp_compact_node": pipeline_amd.COMPACT_NODE_DTYPE,
+    }
+
+    def fields(m):
+        if isinstance(m, np.dtype):
+            return m.itemsize, {f: m.fields[f][1] for f in m.names}
+        return ctypes.sizeof(m), {f: getattr(m, f).offset for f, _ in m._fields_}
+
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "octree_pipeline.h"',
+             "int main(void) {"]
+    for name, m in mirrors.items():
+        lines.append(f'  printf("{name} %zu\\n", sizeof({name}));')
+        for f in fields(m)[1]:
+            lines.append(f'  printf("{name}.{f} %zu\\n", offsetof({name}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-I", os.path.dirname(HEADER),
+                    str(src), "-o", str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True,
+                                                       text=True, check=True).stdout.split("\n")
+               if line)
+    for name, m in mirrors.items():
+        size, offs = fields(m)
+        assert int(got[name]) == size, name
+        for f, off in offs.items():
+            assert int(got[f"{name}.{f}"]) == off, (name, f)
