@@ -424,6 +424,9 @@ typedef struct ocp_tune {
                            10 (three passes over the 30-bit keys; builds with
                            the permutation keep 8) */
   int32_t sort10_kpt;   /* keys per thread of a 10-bit pass (8 or 16) */
+  int32_t oct_order;    /* octree work order in a tile: 0 chains by index,
+                           1 by the size of their group walk (faster on
+                           clustered clouds) */
 } ocp_tune;
 int ocp_get_tune(ocp_ctx ctx, ocp_tune* out);
 int ocp_set_tune(ocp_ctx ctx, const ocp_tune* in);

@@ -346,7 +346,7 @@ SegView single_view(int64_t n, float mn, float rg) {
 
 int tune_key(const TuneTable& t) {
   const int k = ((t.sort_waves * 64 + t.sort_kpt) * 64 + t.morton_ppt) * 64 + t.scan_ipt;
-  return (k * 2 + (t.sort_bits == 10)) * 32 + t.sort10_kpt;
+  return ((k * 2 + (t.sort_bits == 10)) * 32 + t.sort10_kpt) * 2 + (t.oct_order != 0);
 }
 
 int64_t flag_words_for(const TuneTable& tt, int64_t n, int32_t n_seg) {
@@ -525,7 +525,7 @@ bool enqueue_octree(ocp_ctx c, const Plan& p, bool capturing) {
     launch_oct_offsets(kalloc, p.n_seg, w.oct_start, c->ctrl, w.k_cap, s); }
   const OctArgs a = pipeline_oct_args(c);
   KScope k(c, kKOctBuild, capturing);
-  return launch_oct_build(a, p.sv, p.ts_node, p.tiles_node, s);
+  return launch_oct_build(a, p.sv, p.ts_node, p.tiles_node, c->tune.oct_order != 0, s);
 }
 
 void mark(ocp_ctx c, int k, bool capturing) {
@@ -1245,9 +1245,11 @@ int ocp_workspace_size(const ocp_tune* tune, int64_t n, int32_t n_seg, int permu
     tt.scan_ipt = tune->scan_ipt;
     tt.sort_bits = tune->sort_bits;
     tt.sort10_kpt = tune->sort10_kpt;
+    tt.oct_order = tune->oct_order;
     if (!sort_config_supported(tt.sort_waves, tt.sort_kpt) && !(tt.sort_waves == 8 && tt.sort_kpt == 15))
       return OCP_ERR_INVALID;
-    if (!scan_config_supported(tt.scan_ipt) || !sort10_config_supported(tt.sort10_kpt))
+    if (!scan_config_supported(tt.scan_ipt) || !sort10_config_supported(tt.sort10_kpt) ||
+        (tt.oct_order != 0 && tt.oct_order != 1))
       return OCP_ERR_INVALID;
   }
   // The allocations of ensure_workspace, alloc_nodes and ensure_perm for a
@@ -1539,6 +1541,7 @@ int ocp_get_tune(ocp_ctx c, ocp_tune* out) {
   out->scan_ipt = c->tune.scan_ipt;
   out->sort_bits = c->tune.sort_bits;
   out->sort10_kpt = c->tune.sort10_kpt;
+  out->oct_order = c->tune.oct_order;
   return OCP_OK;
 }
 
@@ -1548,7 +1551,8 @@ int ocp_set_tune(ocp_ctx c, const ocp_tune* in) {
                        (in->sort_waves == 8 && in->sort_kpt == 15);
   if (!sort_ok || !morton_config_supported(in->morton_ppt) ||
       !scan_config_supported(in->scan_ipt) || in->morton_grid < 1 ||
-      (in->sort_bits != 8 && in->sort_bits != 10) || !sort10_config_supported(in->sort10_kpt))
+      (in->sort_bits != 8 && in->sort_bits != 10) || !sort10_config_supported(in->sort10_kpt) ||
+      (in->oct_order != 0 && in->oct_order != 1))
     return fail(c, OCP_ERR_INVALID, "unsupported tile configuration");
   OCP_HIP(c, hipStreamSynchronize(c->stream));
   c->tune.sort_waves = in->sort_waves;
@@ -1558,6 +1562,7 @@ int ocp_set_tune(ocp_ctx c, const ocp_tune* in) {
   c->tune.scan_ipt = in->scan_ipt;
   c->tune.sort_bits = in->sort_bits;
   c->tune.sort10_kpt = in->sort10_kpt;
+  c->tune.oct_order = in->oct_order;
   clear_graphs(c);
   c->tab_offsets.clear();
   return OCP_OK;
@@ -1633,6 +1638,11 @@ int ocp_autotune(ocp_ctx c, const float* points, int float3, int64_t n,
         if (int r = try_cfg(t)) return r;
       }
     }
+    for (int order : {0, 1}) {
+      ocp_tune t = best;
+      t.oct_order = order;
+      if (int r = try_cfg(t)) return r;
+    }
     if (std::memcmp(&start, &best, sizeof(best)) == 0) break;
   }
   if (int r = ocp_set_tune(c, &best)) return r;
@@ -1682,10 +1692,11 @@ int ocp_autotune_cached(ocp_ctx c, const char* cache_path, const float* points, 
     bool hit = false;
     while (std::fgets(line, sizeof(line), f)) {
       char k[256];
-      if (std::sscanf(line, "%255s %d %d %d %d %d %d %d", k, &t.sort_waves, &t.sort_kpt,
-                      &t.morton_ppt, &t.morton_grid, &t.scan_ipt, &t.sort_bits,
-                      &t.sort10_kpt) == 8 &&
-          key == k) {
+      t.oct_order = 0;  // lines written before the octree order was tuned
+      const int got = std::sscanf(line, "%255s %d %d %d %d %d %d %d %d", k, &t.sort_waves,
+                                  &t.sort_kpt, &t.morton_ppt, &t.morton_grid, &t.scan_ipt,
+                                  &t.sort_bits, &t.sort10_kpt, &t.oct_order);
+      if ((got == 8 || got == 9) && key == k) {
         found = t;  // the last entry for the class wins
         hit = true;
       }
@@ -1703,9 +1714,9 @@ int ocp_autotune_cached(ocp_ctx c, const char* cache_path, const float* points, 
   if (chosen) *chosen = best;
   FILE* f = std::fopen(cache_path, "a");
   if (!f) return fail(c, OCP_ERR_INVALID, std::string("cannot append to ") + cache_path);
-  std::fprintf(f, "%s %d %d %d %d %d %d %d\n", key.c_str(), best.sort_waves, best.sort_kpt,
-               best.morton_ppt, best.morton_grid, best.scan_ipt, best.sort_bits,
-               best.sort10_kpt);
+  std::fprintf(f, "%s %d %d %d %d %d %d %d %d\n", key.c_str(), best.sort_waves,
+               best.sort_kpt, best.morton_ppt, best.morton_grid, best.scan_ipt, best.sort_bits,
+               best.sort10_kpt, best.oct_order);
   std::fclose(f);
   return OCP_OK;
 }

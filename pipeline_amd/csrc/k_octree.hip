@@ -523,6 +523,11 @@ __device__ __forceinline__ void build_chain(const View& v, int32_t i, int32_t ro
 // scalar registers, which caps it at seven (800 per SIMD); asked for eight
 // it keeps 78 and parks the rest in VGPR lanes. At 10M uniform points:
 // 276 -> 254 us together with the 16-byte window records.
+// BY_GROUP: order the tile's chains by the size of their group walk (see
+// the compaction below); otherwise index order. Which is faster depends on
+// the cloud: at 10M uniform points index order (253 vs 259 us), at 2^24
+// clustered points group order (318 vs 341 us); the autotuner chooses.
+template <bool BY_GROUP>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void oct_build_kernel(
     OctArgs a, SegView sv, const int32_t* __restrict__ tile_start) {
   __shared__ v2i s_pe[kWinN];
@@ -531,6 +536,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
   __shared__ uint8_t s_list[kNodeTile];  // work items as i - i0
   __shared__ uint32_t s_sc[8 * kThreads];
   __shared__ int32_t s_count;
+  __shared__ int32_t s_count_b;  // group order: work items placed from the list's end
   __shared__ int32_t s_emax;
 
   const TileRef r = find_tile(tile_start, sv.n_seg, xcd_tile(blockIdx.x, gridDim.x));
@@ -573,6 +579,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
   const int tid = threadIdx.x;
   if (tid == 0) {
     s_count = 0;
+    if (BY_GROUP) s_count_b = 0;
     s_emax = 0;
   }
   __syncthreads();
@@ -615,17 +622,45 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
   __syncthreads();
   const View v{b, lo, wn, wnc, as_lds<const v2i>(s_pe), as_lds<const v2i>(s_lcc),
                as_lds<uint32_t>(s_sc + tid)};
-  // Compact the block's work: the root and the non-empty chains.
+  // Compact the block's work: the root and the non-empty chains. In group
+  // order, chains none of whose own children is an e == 0 internal node (a
+  // group walk of one member) fill the list from the front and the others
+  // (a child outside the window counts as such, as does the root) from the
+  // back, so waves of the first kind walk one member and the rest walk
+  // their like: at 10M uniform points the slowest lane of a wave walks 4.5
+  // members on average instead of 6.9 in index order.
+  constexpr int kH = kNodeTile / kThreads;
 #pragma unroll
-  for (int h = 0; h < kNodeTile / kThreads; ++h) {
+  for (int h = 0; h < kH; ++h) {
     const int32_t i = i0 + h * kThreads + tid;
     const bool work = (i == 0) || (i < b.m && v.E(i) > 0);
-    const uint64_t bal = __ballot(work);
-    uint32_t base = 0;
-    if (lane_id() == 0 && bal)
-      base = atomicAdd(reinterpret_cast<uint32_t*>(&s_count), static_cast<uint32_t>(__popcll(bal)));
-    base = __shfl(base, 0, 64);
-    if (work) s_list[base + __popcll(bal & lanemask_lt())] = static_cast<uint8_t>(i - i0);
+    bool back = false;
+    if (BY_GROUP && work) {
+      if (i == 0) {
+        back = true;
+      } else {
+        const Rec ri = v.R(i);
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+          const int32_t ch = ri.child() + side;
+          if (ri.leaf(side) || ch <= 0 || ch >= b.m) continue;
+          if (static_cast<uint32_t>(ch - v.lo) >= static_cast<uint32_t>(v.n) || v.E(ch) == 0)
+            back = true;
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < (BY_GROUP ? 2 : 1); ++k) {
+      const bool sel = work && (k == 1) == back;
+      const uint64_t bal = __ballot(sel);
+      uint32_t base = 0;
+      if (lane_id() == 0 && bal)
+        base = atomicAdd(reinterpret_cast<uint32_t*>(k == 0 ? &s_count : &s_count_b),
+                         static_cast<uint32_t>(__popcll(bal)));
+      base = __shfl(base, 0, 64);
+      const uint32_t pos = base + __popcll(bal & lanemask_lt());
+      if (sel) s_list[k == 0 ? pos : kNodeTile - 1 - pos] = static_cast<uint8_t>(i - i0);
+    }
   }
   __syncthreads();
   const int32_t root_level = b.m >= 1 ? v.R(0).pn / 3 : 0;
@@ -633,9 +668,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
   // Chains before the window start no later than P[lo] and end within
   // kMaxEdge slots of their start.
   const int32_t reach_before = (lo > 0 && wn > 0) ? v.P(lo) + kMaxEdge : 0;
-  const int32_t cnt = s_count;
+  const int32_t cnt_a = s_count;
+  const int32_t cnt = BY_GROUP ? cnt_a + s_count_b : cnt_a;
   for (int q = tid; q < cnt; q += kThreads) {
-    const int32_t node = i0 + static_cast<int32_t>(s_list[q]);
+    // (group order: the back items sit at the end of the list)
+    const int32_t slot = (!BY_GROUP || q < cnt_a) ? q : kNodeTile - cnt + q;
+    const int32_t node = i0 + static_cast<int32_t>(s_list[slot]);
     if (node == 0) build_root(v, root_level);
     else build_chain(v, node, root_level, emax_win, reach_before);
   }
@@ -666,14 +704,19 @@ void launch_oct_root(OctNode* nodes, const uint32_t* codes,
 }
 
 bool launch_oct_build(const OctArgs& a, const SegView& sv,
-                      const int32_t* tile_start, int32_t total_tiles, hipStream_t s) {
+                      const int32_t* tile_start, int32_t total_tiles, bool by_group,
+                      hipStream_t s) {
   // The group walk reads the children arrays; the per-segment counts, node
   // bases and the fit flag come from the build.
   if (!a.has_leaf_left || !a.has_leaf_right || !a.left_child || !a.codes ||
       !a.prefix_n || !a.edge || !a.offsets || !a.nodes || !a.ctrl || !a.n_unique ||
       !a.oct_start || !a.kalloc)
     return false;
-  if (total_tiles > 0) oct_build_kernel<<<total_tiles, kThreads, 0, s>>>(a, sv, tile_start);
+  if (total_tiles <= 0) return true;
+  if (by_group)
+    oct_build_kernel<true><<<total_tiles, kThreads, 0, s>>>(a, sv, tile_start);
+  else
+    oct_build_kernel<false><<<total_tiles, kThreads, 0, s>>>(a, sv, tile_start);
   return true;
 }
 

@@ -68,6 +68,7 @@ struct TuneTable {
   int scan_ipt = 16;       // items per thread in unique / edge-scan tiles
   int sort_bits = 8;       // digit width of a build's sort: 8 (4 passes) or 10 (3)
   int sort10_kpt = 8;      // keys per thread of a 10-bit pass (16 waves)
+  int oct_order = 0;       // octree chains in index order (0) or by group size (1)
 };
 
 inline int sort_tile(const TuneTable& t) { return t.sort_waves * 64 * t.sort_kpt; }
@@ -217,7 +218,8 @@ void launch_oct_root(OctNode* nodes, const uint32_t* codes,
 // Needs every OctArgs array (the fused build has them all); returns false
 // (and launches nothing) when an array it needs is null.
 bool launch_oct_build(const OctArgs& a, const SegView& sv,
-                      const int32_t* tile_start, int32_t total_tiles, hipStream_t s);
+                      const int32_t* tile_start, int32_t total_tiles, bool by_group,
+                      hipStream_t s);
 
 // ---- stage 7, per-stage entry points (k_octree_stage.hip)
 // The make call of the reference signature has no children arrays

@@ -206,7 +206,9 @@ def test_tuned_configs_give_identical_outputs(ctx):
                 ((8, 15), (4, 16), (4, 24), (8, 8), (8, 12), (8, 16), (16, 8))] +
                [dict(sort_bits=10, sort10_kpt=k) for k in (8, 16)] +
                [dict(scan_ipt=i) for i in (8, 16, 32)] +
-               [dict(morton_ppt=p, morton_grid=g) for p in (4, 8, 16, 32) for g in (512, 2048)])
+               [dict(morton_ppt=p, morton_grid=g) for p in (4, 8, 16, 32)
+                for g in (256, 512, 2048)] +
+               [dict(oct_order=1)])
     try:
         for cfg in configs:
             ctx.set_tune(**{**base, **cfg})
@@ -221,6 +223,43 @@ def test_tuned_configs_give_identical_outputs(ctx):
         ctx.set_tune(**base)
         d.free()
         db.free()
+
+
+def test_octree_group_order_full_size(ctx):
+    """The octree's other work order (a tile's chains grouped by the size of
+    their group walk) writes the same node bytes as index order on both GPU
+    workloads at full size and on a batch built from a graph."""
+    base = ctx.get_tune()
+    cases = [("uniform 10M", clouds.uniform(10_000_000, seed=114514)),
+             ("clustered 2^24", clouds.clustered(1 << 24, seed=114514))]
+    try:
+        for name, pts in cases:
+            print(f"case {name}", flush=True)
+            d = ctx.upload(pts)
+            got = {}
+            for order in (0, 1):
+                ctx.set_tune(**{**base, "oct_order": order})
+                ctx.build(d, pts.shape[0], 0.0, 1024.0)
+                got[order] = ctx.fetch().nodes[0].view(np.uint8)
+            assert got[0].size > 0
+            np.testing.assert_array_equal(got[1], got[0], err_msg=name)
+            d.free()
+        bpts, offs, smin, srg, _ = _batched_case(40, 1000, 300000, seed=9)
+        db = ctx.upload(bpts)
+        got = {}
+        for order in (0, 1):
+            ctx.set_tune(**{**base, "oct_order": order})
+            ctx.set_graph_mode(True)
+            ctx.build_batched(db, offs, smin, srg)
+            ctx.build_batched(db, offs, smin, srg)
+            out = ctx.fetch(offs)
+            got[order] = [out.nodes[k].view(np.uint8).copy() for k in range(offs.size - 1)]
+        for k in range(offs.size - 1):
+            np.testing.assert_array_equal(got[1][k], got[0][k], err_msg=f"cloud {k}")
+        db.free()
+    finally:
+        ctx.set_graph_mode(False)
+        ctx.set_tune(**base)
 
 
 @pytest.mark.parametrize("kpt", [8, 16])
