@@ -65,10 +65,12 @@ struct Above {
 };
 
 #ifndef OCP_RADIX_HALO
-#define OCP_RADIX_HALO 512
+#define OCP_RADIX_HALO 1024
 #endif
 // Codes staged on either side of a tile. 10M uniform keys: 128 -> 175 us,
-// 256 -> 141 us, 512 -> 131 us (the neighbours' halos mostly hit in L2).
+// 256 -> 141 us, 512 -> 131 us (the neighbours' halos mostly hit in L2);
+// with the wave path for wide nodes, 512 -> 110 us, 1024 -> 105 us,
+// 2048 -> 109 us (2^24 clustered keys: 171, 160, 169 us).
 constexpr int kHalo = OCP_RADIX_HALO;
 // Nodes whose exponential search is still running at this length are
 // finished by a whole wave; the rest stay inside the LDS window.

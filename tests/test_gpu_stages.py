@@ -166,12 +166,13 @@ def test_radix_tree(ctx, case):
         np.testing.assert_array_equal(got[k], exp[k], err_msg=k)
 
 
-@pytest.mark.parametrize("n_codes", [1280, 1281, 1282, 1536, 1537, 1793, 2049])
+@pytest.mark.parametrize("n_codes", [2304, 2305, 2306, 2560, 2561, 2817, 3073])
 def test_radix_tree_interior_tile_threshold(ctx, n_codes):
-    """Tiles of 256 nodes at least 512 nodes from both ends of [0, n] skip
-    the search bound checks. n = n_codes - 1 = 1280 is the first size with
-    such a tile (nodes 512..767); the sizes around it and around the next
-    tiles put the last interior tile right at the threshold."""
+    """Tiles of 256 nodes at least 1024 nodes (twice the local search bound,
+    half the 1024-code halo) from both ends of [0, n] skip the search bound
+    checks. n = n_codes - 1 = 2304 is the first size with such a tile (nodes
+    1024..1279); the sizes around it and around the next tiles put the last
+    interior tile right at the threshold."""
     rng = np.random.default_rng(n_codes)
     codes = np.unique(rng.integers(0, 1 << 30, 3 * n_codes, dtype=np.int64).astype(np.uint32))
     codes = np.sort(rng.choice(codes, n_codes, replace=False))
