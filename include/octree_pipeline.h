@@ -401,7 +401,10 @@ int ocp_get_stage_ms(ocp_ctx ctx, float* ms7);
 /* Per-kernel device time of the last eager build with stage timing on, in
  * launch order, each bracketed by HIP events on the context stream. Writes
  * up to max_kernels entries (names are static strings) and the number of
- * kernels a build launches to n_kernels. */
+ * kernels a build launches to n_kernels. A build with the MSD sort
+ * (ocp_tune.sort_msd) names its sort slots msd_scatter, msd_buckets,
+ * msd_lsd_fallback (msd_lsd_skipped when every bucket fit in LDS) and
+ * unused. */
 int ocp_get_kernel_ms(ocp_ctx ctx, float* ms, const char** names,
                       int32_t max_kernels, int32_t* n_kernels);
 /* Time `steps` builds (after `warmup`) of the configured kind on the
@@ -427,6 +430,12 @@ typedef struct ocp_tune {
   int32_t oct_order;    /* octree work order in a tile: 0 chains by index,
                            1 by the size of their group walk (faster on
                            clustered clouds) */
+  int32_t sort_msd;     /* with sort_bits 10: 0 three LSD passes; 1 one
+                           scatter by the top 10-bit digit, then each bucket
+                           sorted in LDS by one workgroup; a build with a
+                           bucket above 16384 keys runs the three LSD passes
+                           after that, one of more than 2048 clouds only
+                           them */
 } ocp_tune;
 int ocp_get_tune(ocp_ctx ctx, ocp_tune* out);
 int ocp_set_tune(ocp_ctx ctx, const ocp_tune* in);

@@ -63,7 +63,7 @@ class Results(ctypes.Structure):
 class Tune(ctypes.Structure):
     _fields_ = [("sort_waves", _I32), ("sort_kpt", _I32), ("morton_ppt", _I32),
                 ("morton_grid", _I32), ("scan_ipt", _I32), ("sort_bits", _I32),
-                ("sort10_kpt", _I32), ("oct_order", _I32)]
+                ("sort10_kpt", _I32), ("oct_order", _I32), ("sort_msd", _I32)]
 
     def as_dict(self):
         return {f: int(getattr(self, f)) for f, _ in self._fields_}

@@ -16,7 +16,7 @@ LIB = os.path.join(OUT_DIR, "libocpipe.so")
 
 SOURCES = ["k_util.hip", "k_init.hip", "k_morton.hip", "k_onesweep.hip", "k_unique.hip",
            "k_radix_tree.hip", "k_edge_scan.hip", "k_octree.hip", "k_octree_stage.hip",
-           "k_learned.hip", "k_format.hip", "capi.hip"]
+           "k_learned.hip", "k_msd.hip", "k_format.hip", "capi.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("OCP_OFFLOAD_ARCH", "gfx950")

@@ -79,6 +79,10 @@ enum KernelId : int {
 const char* const kKernelNames[kKernelCount] = {
     "begin", "morton_hist", "onesweep_pass0", "onesweep_pass1", "onesweep_pass2",
     "onesweep_pass3", "unique", "radix_tree", "edge_scan", "oct_offsets", "octree"};
+// The sort's kernels when a build takes the MSD form (the last slot unused;
+// the third is named msd_lsd_skipped when every bucket fit in LDS).
+const char* const kMsdKernelNames[4] = {"msd_scatter", "msd_buckets", "msd_lsd_fallback",
+                                        "unused"};
 
 struct LastBuild {
   bool valid = false;
@@ -195,6 +199,7 @@ struct ocp_context {
   hipEvent_t ev[8] = {};
   hipEvent_t kev[2 * kKernelCount] = {};
   bool kran[kKernelCount] = {};  // kernels the last timed eager build ran
+  bool kmsd = false;             // ... with the MSD sort
   bool ev_recorded = false;
   LastBuild last;
   LearnState learn;
@@ -346,7 +351,8 @@ SegView single_view(int64_t n, float mn, float rg) {
 
 int tune_key(const TuneTable& t) {
   const int k = ((t.sort_waves * 64 + t.sort_kpt) * 64 + t.morton_ppt) * 64 + t.scan_ipt;
-  return ((k * 2 + (t.sort_bits == 10)) * 32 + t.sort10_kpt) * 2 + (t.oct_order != 0);
+  return (((k * 2 + (t.sort_bits == 10)) * 32 + t.sort10_kpt) * 2 + (t.oct_order != 0)) * 2 +
+         (t.sort_msd != 0);
 }
 
 int64_t flag_words_for(const TuneTable& tt, int64_t n, int32_t n_seg) {
@@ -545,8 +551,11 @@ bool enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
   // Morton kernel writes the buffer that leaves the sorted keys in keys_a.
   const int bits = pipe_sort_bits(tt, p.permutation);
   const int passes = bits == 10 ? 3 : 4;
-  if (!capturing)
+  const bool msd = pipe_sort_msd(tt, p.permutation, p.n_seg);
+  if (!capturing) {
     for (bool& r : c->kran) r = false;
+    c->kmsd = msd;
+  }
   mark(c, 0, capturing);
   // Begin also clears the per-segment U and K counts (unique leaves an
   // empty segment's entries untouched).
@@ -567,7 +576,24 @@ bool enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
   // on its lowest unfinished tile being resident); the rest may overlap.
   if (c->order_wait && !capturing && c->order_err == hipSuccess)
     c->order_err = hipStreamWaitEvent(s, c->order_wait, 0);
-  for (int pass = 0; pass < passes; ++pass) {
+  if (msd) {
+    // Scatter by the top digit into keys_a, sort each bucket there in LDS;
+    // if a bucket was too large, three LSD passes from the Morton output
+    // (left intact in keys_b) sort into keys_a instead.
+    { KScope k(c, kKPass0, capturing);
+      launch_onesweep_msd(src, dst, p.sv, p.ts_sort, p.tiles_sort, w.hist, w.flags, c->ctrl, tt,
+                          s); }
+    { KScope k(c, kKPass1, capturing);
+      launch_msd_buckets(dst, p.sv, w.hist, c->ctrl, s); }
+    KScope k(c, kKPass2, capturing);
+    const int32_t* run = reinterpret_cast<const int32_t*>(c->ctrl + kCtrlMsdRun);
+    for (int pass = 0; pass < passes; ++pass) {
+      launch_onesweep_pass(src, dst, p.sv, p.ts_sort, p.tiles_sort, w.hist, w.flags, c->ctrl,
+                           pass, pass != 0, tt, s, bits, run);
+      std::swap(src, dst);
+    }
+  }
+  for (int pass = 0; pass < passes && !msd; ++pass) {
     KScope k(c, kKPass0 + pass, capturing);
     if (p.permutation) {
       // Point indices ride with the keys; every pass is stable, so they end
@@ -1246,10 +1272,11 @@ int ocp_workspace_size(const ocp_tune* tune, int64_t n, int32_t n_seg, int permu
     tt.sort_bits = tune->sort_bits;
     tt.sort10_kpt = tune->sort10_kpt;
     tt.oct_order = tune->oct_order;
+    tt.sort_msd = tune->sort_msd;
     if (!sort_config_supported(tt.sort_waves, tt.sort_kpt) && !(tt.sort_waves == 8 && tt.sort_kpt == 15))
       return OCP_ERR_INVALID;
     if (!scan_config_supported(tt.scan_ipt) || !sort10_config_supported(tt.sort10_kpt) ||
-        (tt.oct_order != 0 && tt.oct_order != 1))
+        (tt.oct_order != 0 && tt.oct_order != 1) || (tt.sort_msd != 0 && tt.sort_msd != 1))
       return OCP_ERR_INVALID;
   }
   // The allocations of ensure_workspace, alloc_nodes and ensure_perm for a
@@ -1492,8 +1519,15 @@ int ocp_get_kernel_ms(ocp_ctx c, float* ms, const char** names, int32_t max_kern
   *n_kernels = kKernelCount;
   if (!c->ev_recorded) return fail(c, OCP_ERR_STATE, "no eager build with stage timing on");
   OCP_HIP(c, hipEventSynchronize(c->ev[6]));
+  // Whether the MSD sort's LSD passes had work (read after the build ended).
+  uint32_t msd_run = 0;
+  if (c->kmsd && names)
+    OCP_HIP(c, hipMemcpy(&msd_run, c->ctrl + kCtrlMsdRun, sizeof(msd_run), hipMemcpyDeviceToHost));
   for (int k = 0; k < kKernelCount && k < max_kernels; ++k) {
-    if (names) names[k] = kKernelNames[k];
+    if (names)
+      names[k] = !(c->kmsd && k >= kKPass0 && k <= kKPass3) ? kKernelNames[k]
+                 : (k == kKPass2 && msd_run == 0)           ? "msd_lsd_skipped"
+                                                            : kMsdKernelNames[k - kKPass0];
     if (!ms) continue;
     ms[k] = 0.0f;  // a kernel the build did not launch (a fourth sort pass)
     if (c->kran[k]) OCP_HIP(c, hipEventElapsedTime(&ms[k], c->kev[2 * k], c->kev[2 * k + 1]));
@@ -1542,6 +1576,7 @@ int ocp_get_tune(ocp_ctx c, ocp_tune* out) {
   out->sort_bits = c->tune.sort_bits;
   out->sort10_kpt = c->tune.sort10_kpt;
   out->oct_order = c->tune.oct_order;
+  out->sort_msd = c->tune.sort_msd;
   return OCP_OK;
 }
 
@@ -1552,7 +1587,7 @@ int ocp_set_tune(ocp_ctx c, const ocp_tune* in) {
   if (!sort_ok || !morton_config_supported(in->morton_ppt) ||
       !scan_config_supported(in->scan_ipt) || in->morton_grid < 1 ||
       (in->sort_bits != 8 && in->sort_bits != 10) || !sort10_config_supported(in->sort10_kpt) ||
-      (in->oct_order != 0 && in->oct_order != 1))
+      (in->oct_order != 0 && in->oct_order != 1) || (in->sort_msd != 0 && in->sort_msd != 1))
     return fail(c, OCP_ERR_INVALID, "unsupported tile configuration");
   OCP_HIP(c, hipStreamSynchronize(c->stream));
   c->tune.sort_waves = in->sort_waves;
@@ -1563,6 +1598,7 @@ int ocp_set_tune(ocp_ctx c, const ocp_tune* in) {
   c->tune.sort_bits = in->sort_bits;
   c->tune.sort10_kpt = in->sort10_kpt;
   c->tune.oct_order = in->oct_order;
+  c->tune.sort_msd = in->sort_msd;
   clear_graphs(c);
   c->tab_offsets.clear();
   return OCP_OK;
@@ -1595,7 +1631,8 @@ int ocp_autotune(ocp_ctx c, const float* points, int float3, int64_t n,
   float best_ms = 0;
   if (int r = time_cfg(best, &best_ms)) return r;
   // Coordinate descent, two rounds: sort tile of 8-bit digits, then the
-  // pass count (four 8-bit passes or three 10-bit ones, two tile sizes),
+  // pass count (four 8-bit passes, three 10-bit ones, or the MSD form: one
+  // 10-bit scatter and LDS bucket sorts; two tile sizes each),
   // then scan tile, then Morton tile/grid. The second round revisits the
   // pass count under the chosen Morton shape: the Morton kernel's histogram
   // flush grows with its grid and three times over with 10-bit digits, so
@@ -1619,11 +1656,14 @@ int ocp_autotune(ocp_ctx c, const float* points, int float3, int64_t n,
       t.sort_kpt = sc[1];
       if (int r = try_cfg(t)) return r;
     }
-    for (int kpt : {8, 16}) {
-      ocp_tune t = best;
-      t.sort_bits = 10;
-      t.sort10_kpt = kpt;
-      if (int r = try_cfg(t)) return r;
+    for (int msd : {0, 1}) {
+      for (int kpt : {8, 16}) {
+        ocp_tune t = best;
+        t.sort_bits = 10;
+        t.sort10_kpt = kpt;
+        t.sort_msd = msd;
+        if (int r = try_cfg(t)) return r;
+      }
     }
     for (int ipt : {8, 16, 32}) {
       ocp_tune t = best;
@@ -1693,10 +1733,11 @@ int ocp_autotune_cached(ocp_ctx c, const char* cache_path, const float* points, 
     while (std::fgets(line, sizeof(line), f)) {
       char k[256];
       t.oct_order = 0;  // lines written before the octree order was tuned
-      const int got = std::sscanf(line, "%255s %d %d %d %d %d %d %d %d", k, &t.sort_waves,
+      t.sort_msd = 0;   // ... or the MSD sort
+      const int got = std::sscanf(line, "%255s %d %d %d %d %d %d %d %d %d", k, &t.sort_waves,
                                   &t.sort_kpt, &t.morton_ppt, &t.morton_grid, &t.scan_ipt,
-                                  &t.sort_bits, &t.sort10_kpt, &t.oct_order);
-      if ((got == 8 || got == 9) && key == k) {
+                                  &t.sort_bits, &t.sort10_kpt, &t.oct_order, &t.sort_msd);
+      if (got >= 8 && key == k) {
         found = t;  // the last entry for the class wins
         hit = true;
       }
@@ -1714,9 +1755,9 @@ int ocp_autotune_cached(ocp_ctx c, const char* cache_path, const float* points, 
   if (chosen) *chosen = best;
   FILE* f = std::fopen(cache_path, "a");
   if (!f) return fail(c, OCP_ERR_INVALID, std::string("cannot append to ") + cache_path);
-  std::fprintf(f, "%s %d %d %d %d %d %d %d %d\n", key.c_str(), best.sort_waves,
+  std::fprintf(f, "%s %d %d %d %d %d %d %d %d %d\n", key.c_str(), best.sort_waves,
                best.sort_kpt, best.morton_ppt, best.morton_grid, best.scan_ipt, best.sort_bits,
-               best.sort10_kpt, best.oct_order);
+               best.sort10_kpt, best.oct_order, best.sort_msd);
   std::fclose(f);
   return OCP_OK;
 }

@@ -34,6 +34,7 @@
 //      (host side); constant buckets need nothing;
 //   6. every bucket boundary is checked (a misorder makes the host re-sort
 //      the whole array with OneSweep).
+#include "bucket_sort.hpp"
 #include "learned.hpp"
 #include "onesweep_impl.hpp"
 
@@ -385,154 +386,16 @@ __global__ __launch_bounds__(1024) void lprefix_kernel(LearnTables tb, int32_t f
 }
 
 // ------------------------------------------------------------ bucket sort
-// One workgroup per bucket whose size is in (lo_cap, hi_cap]: keys are
-// loaded wave-contiguously, taken relative to the bucket minimum, and
-// sorted by stable LSD passes of 8 bits over the bits they span, ranked
-// per wave by ballot multisplit and scattered through LDS.
+// One workgroup per bucket whose size is in (lo_cap, WAVES * 64 * KPT].
 template <int WAVES, int KPT>
 __global__ __launch_bounds__(WAVES * 64) void lsort_kernel(uint32_t* __restrict__ keys,
                                                            const int32_t* __restrict__ bstart,
                                                            int32_t lo_cap) {
-  constexpr int THREADS = WAVES * 64;
-  constexpr int CAP = THREADS * KPT;
-  __shared__ uint32_t s_keys[CAP];
-  __shared__ uint32_t s_whist[WAVES * 256];
-  __shared__ uint32_t s_dstart[256];
-  __shared__ uint32_t s_tmp[WAVES];
-  __shared__ uint32_t s_mm[2];
   const int32_t b = blockIdx.x;
   const int32_t lo = bstart[b];
   const int32_t n = bstart[b + 1] - lo;
-  if (n <= lo_cap || n > CAP) return;
-  const int tid = threadIdx.x;
-  const uint32_t w = wave_id();
-  const uint32_t lane = lane_id();
-  uint32_t* g = keys + lo;
-  if (tid == 0) {
-    s_mm[0] = 0xffffffffu;
-    s_mm[1] = 0u;
-  }
-  uint32_t kv[KPT];
-  uint32_t mn = 0xffffffffu, mx = 0u;
-  // All loads first (consuming each in its own iteration keeps one load in
-  // flight per wave), then the bucket's range.
-#pragma unroll
-  for (int k = 0; k < KPT; ++k) {
-    const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
-    kv[k] = idx < n ? g[idx] : 0u;
-  }
-#pragma unroll
-  for (int k = 0; k < KPT; ++k) {
-    const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
-    if (idx < n) {
-      mn = min(mn, kv[k]);
-      mx = max(mx, kv[k]);
-    }
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    mn = min(mn, static_cast<uint32_t>(__shfl_xor(static_cast<int>(mn), o, 64)));
-    mx = max(mx, static_cast<uint32_t>(__shfl_xor(static_cast<int>(mx), o, 64)));
-  }
-  __syncthreads();
-  if (lane == 0) {
-    atomicMin(&s_mm[0], mn);
-    atomicMax(&s_mm[1], mx);
-  }
-  __syncthreads();
-  mn = s_mm[0];
-  mx = s_mm[1];
-  if (mn == mx) return;  // constant bucket: already sorted
-  const int bits = 32 - __clz(static_cast<int>(mx - mn));
-  const int passes = (bits + 7) / 8;
-#pragma unroll
-  for (int k = 0; k < KPT; ++k) {
-    const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
-    kv[k] = idx < n ? kv[k] - mn : 0xffffffffu;  // padding ranks last
-  }
-  const uint64_t lt = lanemask_lt();
-  uint32_t* wh = s_whist + w * 256;
-  // Pass 0 needs no stability (no earlier order to keep), so it is a plain
-  // counting sort with LDS atomics; padding stays out of it and is parked at
-  // the top of the array, where the stable passes keep it last.
-  {
-    for (int i = tid; i < 256; i += THREADS) s_whist[i] = 0;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < KPT; ++k) {
-      const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
-      if (idx < n) atomicAdd(&s_whist[kv[k] & 255u], 1u);
-    }
-    __syncthreads();
-    const uint32_t cnt = tid < 256 ? s_whist[tid] : 0u;
-    uint32_t all;
-    const uint32_t ds = block_excl_scan<THREADS>(cnt, s_tmp, &all);
-    if (tid < 256) s_whist[tid] = ds;  // cursor per digit
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < KPT; ++k) {
-      const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
-      if (idx < n) s_keys[atomicAdd(&s_whist[kv[k] & 255u], 1u)] = kv[k];
-    }
-    for (int i = n + tid; i < CAP; i += THREADS) s_keys[i] = 0xffffffffu;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < KPT; ++k)
-      kv[k] = s_keys[static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane)];
-    __syncthreads();
-  }
-  for (int pass = 1; pass < passes; ++pass) {
-    const uint32_t shift = 8u * static_cast<uint32_t>(pass);
-    for (int i = tid; i < WAVES * 256; i += THREADS) s_whist[i] = 0;
-    __syncthreads();
-    uint32_t off[KPT];
-#pragma unroll
-    for (int k = 0; k < KPT; ++k) {
-      const uint32_t d = (kv[k] >> shift) & 255u;
-      uint64_t msk = ~0ull;
-#pragma unroll
-      for (int bb = 0; bb < 8; ++bb) {
-        const bool bit = (d >> bb) & 1u;
-        const uint64_t bal = __ballot(bit);
-        msk &= bit ? bal : ~bal;
-      }
-      const uint32_t rank = __popcll(msk & lt);
-      const uint32_t old = wh[d];
-      if (rank == 0) wh[d] = old + __popcll(msk);
-      off[k] = old + rank;
-    }
-    __syncthreads();
-    uint32_t tot = 0;
-    if (tid < 256) {
-      uint32_t run = 0;
-#pragma unroll
-      for (int ww = 0; ww < WAVES; ++ww) {
-        const uint32_t cc = s_whist[ww * 256 + tid];
-        s_whist[ww * 256 + tid] = run;
-        run += cc;
-      }
-      tot = run;
-    }
-    uint32_t all;
-    const uint32_t ds = block_excl_scan<THREADS>(tot, s_tmp, &all);
-    if (tid < 256) s_dstart[tid] = ds;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < KPT; ++k) {
-      const uint32_t d = (kv[k] >> shift) & 255u;
-      s_keys[s_dstart[d] + wh[d] + off[k]] = kv[k];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < KPT; ++k)
-      kv[k] = s_keys[static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane)];
-    __syncthreads();
-  }
-#pragma unroll
-  for (int k = 0; k < KPT; ++k) {
-    const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
-    if (idx < n) g[idx] = kv[k] + mn;
-  }
+  if (n <= lo_cap || n > WAVES * 64 * KPT) return;
+  lds_bucket_sort<WAVES, KPT>(keys + lo, n);
 }
 
 // The last key of each non-empty bucket must not exceed the next key.

@@ -91,18 +91,20 @@ void launch_cfg(const uint32_t* in, uint32_t* out, const SegView& sv,
 }
 
 // 10-bit digits: one thread per digit, so 16 waves per tile.
+// `digit` selects the bits (10 * digit) and the histogram slot; `stage` tags
+// the lookback flags; tiles_dev (optional) is the device-side tile count.
 template <int KPT>
 void launch_cfg10(const uint32_t* in, uint32_t* out, const SegView& sv,
                   const int32_t* tile_start, int32_t total_tiles, const uint32_t* hist,
-                  uint64_t* flags, uint32_t* ctrl, int pass, bool stable, hipStream_t s) {
-  const uint32_t stage = kStagePass0 + static_cast<uint32_t>(pass);
-  const RadixDigit10 dig{10u * static_cast<uint32_t>(pass)};
+                  uint64_t* flags, uint32_t* ctrl, int digit, uint32_t stage, bool stable,
+                  const int32_t* tiles_dev, hipStream_t s) {
+  const RadixDigit10 dig{10u * static_cast<uint32_t>(digit)};
   if (stable)
     onesweep_kernel<16, KPT, RadixDigit10, true><<<total_tiles, 1024, 0, s>>>(
-        in, out, sv, tile_start, nullptr, hist, pass, flags, ctrl, stage, dig);
+        in, out, sv, tile_start, tiles_dev, hist, digit, flags, ctrl, stage, dig);
   else
     onesweep_kernel<16, KPT, RadixDigit10, false><<<total_tiles, 1024, 0, s>>>(
-        in, out, sv, tile_start, nullptr, hist, pass, flags, ctrl, stage, dig);
+        in, out, sv, tile_start, tiles_dev, hist, digit, flags, ctrl, stage, dig);
 }
 
 template <int WAVES, int KPT>
@@ -152,13 +154,17 @@ void launch_onesweep_pass(const uint32_t* in, uint32_t* out,
                           const SegView& sv, const int32_t* tile_start,
                           int32_t total_tiles, const uint32_t* hist,
                           uint64_t* flags, uint32_t* ctrl, int pass, bool stable,
-                          const TuneTable& tt, hipStream_t s, int bits) {
+                          const TuneTable& tt, hipStream_t s, int bits,
+                          const int32_t* tiles_dev) {
   if (total_tiles <= 0) return;
   if (bits == 10) {
+    const uint32_t stage = kStagePass0 + static_cast<uint32_t>(pass);
     if (tt.sort10_kpt == 16)
-      launch_cfg10<16>(in, out, sv, tile_start, total_tiles, hist, flags, ctrl, pass, stable, s);
+      launch_cfg10<16>(in, out, sv, tile_start, total_tiles, hist, flags, ctrl, pass, stage,
+                       stable, tiles_dev, s);
     else
-      launch_cfg10<8>(in, out, sv, tile_start, total_tiles, hist, flags, ctrl, pass, stable, s);
+      launch_cfg10<8>(in, out, sv, tile_start, total_tiles, hist, flags, ctrl, pass, stage,
+                      stable, tiles_dev, s);
     return;
   }
 #define OCP_SORT_CFG(W, K)                                                   \
@@ -176,6 +182,21 @@ void launch_onesweep_pass(const uint32_t* in, uint32_t* out,
 #undef OCP_SORT_CFG
   launch_cfg<8, 15>(in, out, sv, tile_start, total_tiles, hist, flags, ctrl,
                     pass, stable, s);
+}
+
+void launch_onesweep_msd(const uint32_t* in, uint32_t* out, const SegView& sv,
+                         const int32_t* tile_start, int32_t total_tiles, const uint32_t* hist,
+                         uint64_t* flags, uint32_t* ctrl, const TuneTable& tt, hipStream_t s) {
+  if (total_tiles <= 0) return;
+  // The top digit of a 30-bit key, unranked among equals (the buckets are
+  // sorted afterwards), under the stage id no 10-bit LSD pass uses.
+  constexpr uint32_t stage = kStagePass0 + 3;
+  if (tt.sort10_kpt == 16)
+    launch_cfg10<16>(in, out, sv, tile_start, total_tiles, hist, flags, ctrl, 2, stage, false,
+                     nullptr, s);
+  else
+    launch_cfg10<8>(in, out, sv, tile_start, total_tiles, hist, flags, ctrl, 2, stage, false,
+                    nullptr, s);
 }
 
 void launch_histogram(const uint32_t* keys, const SegView& sv,

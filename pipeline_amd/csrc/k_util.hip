@@ -23,6 +23,7 @@ __global__ __launch_bounds__(256) void begin_kernel(uint32_t* ctrl,
   if (tid == 0) {
     ctrl[kCtrlEpoch] = ctrl[kCtrlEpoch] + 1;
     ctrl[kCtrlLearnBad] = 0;
+    ctrl[kCtrlMsdRun] = 0;
   }
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t i = tid; i < n_hist; i += stride) hist[i] = 0;

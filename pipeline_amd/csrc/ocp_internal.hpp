@@ -16,7 +16,7 @@ enum CtrlSlot : int {
   kCtrlEpoch = 8,      // bumped once per launch group; tags lookback flags
   kCtrlOctTotal = 9,   // total octree slots of the batch
   kCtrlOctOk = 10,     // 1 when the octree fits the node capacity
-  // word 11 is unused
+  kCtrlMsdRun = 11,    // MSD sort: 0, or 0x7fffffff when the LSD passes must run
   kCtrlLearnBad = 12,  // learned sort: misordered bucket boundaries
   kCtrlTimeout = 13,   // mark of the launch whose wait gave up (0: none)
   kCtrlWaitLimit = 14, // 0: waits poll up to kSpinLimit times; v > 0: v - 1
@@ -69,6 +69,7 @@ struct TuneTable {
   int sort_bits = 8;       // digit width of a build's sort: 8 (4 passes) or 10 (3)
   int sort10_kpt = 8;      // keys per thread of a 10-bit pass (16 waves)
   int oct_order = 0;       // octree chains in index order (0) or by group size (1)
+  int sort_msd = 0;        // 10-bit digits: LSD passes (0) or MSD scatter + LDS buckets (1)
 };
 
 inline int sort_tile(const TuneTable& t) { return t.sort_waves * 64 * t.sort_kpt; }
@@ -76,6 +77,12 @@ inline int sort_tile(const TuneTable& t) { return t.sort_waves * 64 * t.sort_kpt
 // the 30-bit Morton keys) when chosen and no permutation rides along.
 inline int pipe_sort_bits(const TuneTable& t, bool perm) {
   return (t.sort_bits == 10 && !perm) ? 10 : 8;
+}
+// The MSD form: chosen, 10-bit digits, and few enough segments for one
+// workgroup per (segment, bucket).
+constexpr int32_t kMsdMaxSegments = 2048;  // grid: n_seg * 1024 workgroups of 1024
+inline bool pipe_sort_msd(const TuneTable& t, bool perm, int32_t n_seg) {
+  return t.sort_msd != 0 && pipe_sort_bits(t, perm) == 10 && n_seg <= kMsdMaxSegments;
 }
 inline int pipe_sort_tile(const TuneTable& t, bool perm) {
   return pipe_sort_bits(t, perm) == 10 ? 1024 * t.sort10_kpt : sort_tile(t);
@@ -150,7 +157,18 @@ void launch_onesweep_pass(const uint32_t* in, uint32_t* out,
                           const SegView& sv, const int32_t* tile_start,
                           int32_t total_tiles, const uint32_t* hist,
                           uint64_t* flags, uint32_t* ctrl, int pass, bool stable,
-                          const TuneTable& tt, hipStream_t s, int bits = 8);
+                          const TuneTable& tt, hipStream_t s, int bits = 8,
+                          const int32_t* tiles_dev = nullptr);  // (10-bit passes)
+// MSD sort of bare 30-bit keys (k_msd.hip): the scatter by the top 10-bit
+// digit (histogram slot 2 of the 10-bit histograms), then one workgroup per
+// (segment, bucket) sorting it in LDS in place; a segment with a bucket too
+// large for LDS raises ctrl[kCtrlMsdRun] instead, the tile count of the LSD
+// passes that then sort everything again. Grid: n_seg * 1024 workgroups.
+void launch_onesweep_msd(const uint32_t* in, uint32_t* out, const SegView& sv,
+                         const int32_t* tile_start, int32_t total_tiles, const uint32_t* hist,
+                         uint64_t* flags, uint32_t* ctrl, const TuneTable& tt, hipStream_t s);
+void launch_msd_buckets(uint32_t* keys, const SegView& sv, const uint32_t* hist, uint32_t* ctrl,
+                        hipStream_t s);
 
 // Stable pass carrying a value with every key (vin == nullptr: the key's
 // index in its segment).

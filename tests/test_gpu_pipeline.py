@@ -204,7 +204,7 @@ def test_tuned_configs_give_identical_outputs(ctx):
     base = ctx.get_tune()
     configs = ([dict(sort_waves=w, sort_kpt=k) for w, k in
                 ((8, 15), (4, 16), (4, 24), (8, 8), (8, 12), (8, 16), (16, 8))] +
-               [dict(sort_bits=10, sort10_kpt=k) for k in (8, 16)] +
+               [dict(sort_bits=10, sort10_kpt=k, sort_msd=m) for k in (8, 16) for m in (0, 1)] +
                [dict(scan_ipt=i) for i in (8, 16, 32)] +
                [dict(morton_ppt=p, morton_grid=g) for p in (4, 8, 16, 32)
                 for g in (256, 512, 2048)] +
@@ -258,6 +258,74 @@ def test_octree_group_order_full_size(ctx):
             np.testing.assert_array_equal(got[1][k], got[0][k], err_msg=f"cloud {k}")
         db.free()
     finally:
+        ctx.set_graph_mode(False)
+        ctx.set_tune(**base)
+
+
+def _msd_fallback_ran(ctx):
+    names = list(ctx.kernel_ms())
+    assert "msd_scatter" in names and "msd_buckets" in names
+    assert ("msd_lsd_fallback" in names) != ("msd_lsd_skipped" in names)
+    return "msd_lsd_fallback" in names
+
+
+def test_msd_sort_full_size(ctx):
+    """The MSD form of the sort (one 10-bit scatter by the top digit, then
+    every bucket sorted in LDS by one workgroup) gives the oracle's bytes on
+    10M uniform points, where every bucket fits in LDS and the LSD passes
+    are skipped, and on 2^24 clustered points, whose large buckets send the
+    build through the three LSD passes; then on a batch that holds both
+    kinds of cloud (eager and from a graph, twice each) and on a batch of
+    uniform clouds, all of whose buckets fit; builds with the permutation
+    keep stable 8-bit passes."""
+    base = ctx.get_tune()
+    try:
+        ctx.set_tune(**{**base, "sort_bits": 10, "sort_msd": 1})
+        ctx.set_stage_timing(True)
+        for name, pts, fallback in (
+                ("uniform 10M", clouds.uniform(10_000_000, seed=114514), False),
+                ("clustered 2^24", clouds.clustered(1 << 24, seed=114514), True)):
+            print(f"case {name}", flush=True)
+            d = ctx.upload(pts)
+            ctx.build(d, pts.shape[0], 0.0, 1024.0)
+            assert _msd_fallback_ran(ctx) == fallback, name
+            _check_segment(ctx.fetch(), 0, oracle.build_plain(pts))
+            d.free()
+        sizes = [1, 777, 65536, 300_000, 2_000_000]
+        upts = np.concatenate([clouds.uniform(n, seed=40 + k) for k, n in enumerate(sizes)])
+        uoffs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+        ubox = (np.zeros(len(sizes), np.float32), np.full(len(sizes), 1024.0, np.float32))
+        for name, (bpts, offs, smin, srg), fallback in (
+                ("mixed batch", _batched_case(10, 1000, 300_000, seed=31)[:4], None),
+                ("uniform batch", (upts, uoffs) + ubox, False)):
+            print(f"case {name}", flush=True)
+            exps = [oracle.build_plain(bpts[offs[s]:offs[s + 1]], float(smin[s]), float(srg[s]))
+                    for s in range(offs.size - 1)]
+            db = ctx.upload(bpts)
+            for graph in (False, True):
+                ctx.set_graph_mode(graph)
+                for _ in range(2):
+                    ctx.build_batched(db, offs, smin, srg)
+                    if not graph and fallback is not None:
+                        assert _msd_fallback_ran(ctx) == fallback, name
+                    out = ctx.fetch(offs)
+                    for s in range(offs.size - 1):
+                        _check_segment(out, s, exps[s])
+            ctx.set_graph_mode(False)
+            db.free()
+        ctx.set_stage_timing(False)
+        small = clouds.uniform(300_000, seed=5)
+        ds = ctx.upload(small)
+        ctx.set_permutation(True)
+        ctx.build(ds, small.shape[0], 0.0, 1024.0)
+        out = ctx.fetch()
+        keys = oracle.morton(small, 0.0, 1024.0)
+        np.testing.assert_array_equal(out.sorted_index[0], np.argsort(keys, kind="stable"))
+        _check_segment(out, 0, oracle.build_plain(small))
+        ds.free()
+    finally:
+        ctx.set_stage_timing(False)
+        ctx.set_permutation(False)
         ctx.set_graph_mode(False)
         ctx.set_tune(**base)
 
