@@ -14,12 +14,17 @@ namespace ocp {
 // 8-bit passes ranked per wave by ballot multisplit - scattered through LDS.
 // Keys spanning 20 bits take two passes. Every thread of the workgroup
 // calls it.
-template <int WAVES, int KPT>
+template <int WAVES, int KPT, int P0 = (WAVES >= 16 ? 12 : WAVES == 8 ? 11 : 10)>
 __device__ __forceinline__ void lds_bucket_sort(uint32_t* __restrict__ g, int32_t n) {
   constexpr int THREADS = WAVES * 64;
   constexpr int CAP = THREADS * KPT;
+  // First-pass counters (2^P0) and the stable passes' per-wave digit
+  // counters share one array.
+  constexpr int NB = 1 << P0;
+  constexpr int HW = NB > WAVES * 256 ? NB : WAVES * 256;
+  static_assert(P0 >= 8 && NB % THREADS == 0, "whole counters per thread");
   __shared__ uint32_t s_keys[CAP];
-  __shared__ uint32_t s_whist[WAVES * 256];
+  __shared__ uint32_t s_whist[HW];
   __shared__ uint32_t s_dstart[256];
   __shared__ uint32_t s_tmp[WAVES];
   __shared__ uint32_t s_mm[2];
@@ -77,9 +82,7 @@ __device__ __forceinline__ void lds_bucket_sort(uint32_t* __restrict__ g, int32_
   // counting sort with LDS atomics over the low P0 bits, one counter per
   // word of s_whist; padding stays out of it, and the rest of the last
   // partial row is filled with it, where the stable passes keep it last.
-  constexpr int NB = WAVES * 256;
-  constexpr int P0 = NB == 4096 ? 12 : NB == 2048 ? 11 : NB == 1024 ? 10 : 0;
-  static_assert(P0 != 0 && NB == 4 * THREADS, "4, 8 or 16 waves");
+  constexpr int PER = NB / THREADS;
   {
     for (int i = tid; i < NB; i += THREADS) s_whist[i] = 0;
     __syncthreads();
@@ -90,18 +93,18 @@ __device__ __forceinline__ void lds_bucket_sort(uint32_t* __restrict__ g, int32_
       if (idx < n) atomicAdd(&s_whist[kv[k] & (NB - 1u)], 1u);
     }
     __syncthreads();
-    uint32_t cc[4];
+    uint32_t cc[PER];
     uint32_t sum = 0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      cc[j] = s_whist[tid * 4 + j];
+    for (int j = 0; j < PER; ++j) {
+      cc[j] = s_whist[tid * PER + j];
       sum += cc[j];
     }
     uint32_t all;
     uint32_t run = block_excl_scan<THREADS>(sum, s_tmp, &all);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      s_whist[tid * 4 + j] = run;  // cursor per digit
+    for (int j = 0; j < PER; ++j) {
+      s_whist[tid * PER + j] = run;  // cursor per digit
       run += cc[j];
     }
     __syncthreads();

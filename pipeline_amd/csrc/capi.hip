@@ -61,6 +61,7 @@ struct Workspace {
 struct Plan {
   int32_t n_seg = 1;
   int64_t n_total = 0;
+  int64_t max_seg = 0;        // the largest segment's length
   bool device_boxes = false;  // boxes reduced on the device into ws.boxes
   bool permutation = false;   // point indices carried through sort and unique
   SegView sv{};
@@ -370,6 +371,7 @@ Plan plain_plan(const TuneTable& tt, int64_t n, float mn, float rg, bool perm) {
   Plan p;
   p.n_seg = 1;
   p.n_total = n;
+  p.max_seg = n;
   p.permutation = perm;
   p.sv = single_view(n, mn, rg);
   p.tiles_morton = static_cast<int32_t>(cdiv(n, morton_tile(tt)));
@@ -431,6 +433,7 @@ int batched_plan(ocp_ctx c, const int32_t* offsets, const float* smin,
     host[4 * (n_seg + 1) + k] = static_cast<int32_t>(tn);
     if (k < n_seg) {
       const int64_t len = int64_t(offsets[k + 1]) - offsets[k];
+      p.max_seg = std::max(p.max_seg, len);
       tm += cdiv(len, morton_tile(tt));
       ts += cdiv(len, stile);
       tc += cdiv(len, scan_tile(tt));
@@ -568,7 +571,8 @@ bool enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
   uint32_t* src = (passes % 2) ? w.keys_b : w.keys_a;
   uint32_t* dst = (passes % 2) ? w.keys_a : w.keys_b;
   { KScope k(c, kKMorton, capturing);
-    launch_morton(pts, f3, src, p.sv, p.ts_morton, p.tiles_morton, w.hist, tt, s, bits); }
+    launch_morton(pts, f3, src, p.sv, p.ts_morton, p.tiles_morton, w.hist, tt, s,
+                  msd ? kHistTop10 : bits); }
   mark(c, 1, capturing);
   uint32_t* vsrc = nullptr;  // the first pass numbers the points itself
   uint32_t* vdst = w.vals_b;
@@ -578,14 +582,16 @@ bool enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
     c->order_err = hipStreamWaitEvent(s, c->order_wait, 0);
   if (msd) {
     // Scatter by the top digit into keys_a, sort each bucket there in LDS;
-    // if a bucket was too large, three LSD passes from the Morton output
-    // (left intact in keys_b) sort into keys_a instead.
+    // if a bucket was too large, the counts of the two lower digits and
+    // three LSD passes from the Morton output (left intact in keys_b) sort
+    // into keys_a instead.
     { KScope k(c, kKPass0, capturing);
       launch_onesweep_msd(src, dst, p.sv, p.ts_sort, p.tiles_sort, w.hist, w.flags, c->ctrl, tt,
                           s); }
     { KScope k(c, kKPass1, capturing);
-      launch_msd_buckets(dst, p.sv, w.hist, c->ctrl, s); }
+      launch_msd_buckets(dst, p.sv, w.hist, c->ctrl, p.max_seg, s); }
     KScope k(c, kKPass2, capturing);
+    launch_msd_low_hist(src, p.sv, p.ts_morton, p.tiles_morton, w.hist, c->ctrl, tt, s);
     const int32_t* run = reinterpret_cast<const int32_t*>(c->ctrl + kCtrlMsdRun);
     for (int pass = 0; pass < passes; ++pass) {
       launch_onesweep_pass(src, dst, p.sv, p.ts_sort, p.tiles_sort, w.hist, w.flags, c->ctrl,

@@ -16,7 +16,8 @@ namespace ocp {
 namespace {
 
 // HB: digit width of the fused histograms (0: none; 8: 4 x 256 bins; 10:
-// 3 x 1024 bins, for keys below 2^30).
+// 3 x 1024 bins, for keys below 2^30; kHistTop10: the third of those only,
+// the top digit that the MSD sort scatters by).
 template <int PPT, bool F3, int HB>
 __global__ __launch_bounds__(256) void morton_kernel(
     const float* __restrict__ pts, uint32_t* __restrict__ keys, SegView sv,
@@ -24,7 +25,10 @@ __global__ __launch_bounds__(256) void morton_kernel(
     uint32_t* __restrict__ hist) {
   constexpr int T = 256 * PPT;
   constexpr bool HIST = HB != 0;
-  constexpr int HW = HB == 10 ? 3072 : 1024;  // histogram words per segment
+  constexpr bool TOP = HB == kHistTop10;
+  constexpr int HW = HB == 10 ? 3072 : 1024;   // LDS histogram words
+  constexpr int HS = HB == 8 ? 1024 : 3072;    // histogram words per segment
+  constexpr int HO = TOP ? 2 * 1024 : 0;       // where the LDS words go in them
   __shared__ uint32_t s_hist[HIST ? HW : 1];
   const int tid = threadIdx.x;
   const int32_t per = (total_tiles + gridDim.x - 1) / gridDim.x;
@@ -40,7 +44,7 @@ __global__ __launch_bounds__(256) void morton_kernel(
     for (int d = tid; d < HW; d += 256) {
       const uint32_t v = s_hist[d];
       if (v) {
-        atomicAdd(&hist[static_cast<int64_t>(seg) * HW + d], v);
+        atomicAdd(&hist[static_cast<int64_t>(seg) * HS + HO + d], v);
         s_hist[d] = 0;
       }
     }
@@ -67,6 +71,8 @@ __global__ __launch_bounds__(256) void morton_kernel(
         atomicAdd(&s_hist[key & 1023u], 1u);
         atomicAdd(&s_hist[1024 + ((key >> 10) & 1023u)], 1u);
         atomicAdd(&s_hist[2048 + ((key >> 20) & 1023u)], 1u);
+      } else if (TOP) {
+        atomicAdd(&s_hist[(key >> 20) & 1023u], 1u);
       }
     };
     if (cnt == T) {
@@ -116,6 +122,9 @@ void launch_f3(const float* pts, uint32_t* keys, const SegView& sv, const int32_
     morton_kernel<PPT, F3, 0><<<grid, 256, 0, s>>>(pts, keys, sv, tile_start, total_tiles, hist);
   else if (hist_bits == 10)
     morton_kernel<PPT, F3, 10><<<grid, 256, 0, s>>>(pts, keys, sv, tile_start, total_tiles, hist);
+  else if (hist_bits == kHistTop10)
+    morton_kernel<PPT, F3, kHistTop10><<<grid, 256, 0, s>>>(pts, keys, sv, tile_start,
+                                                            total_tiles, hist);
   else
     morton_kernel<PPT, F3, 8><<<grid, 256, 0, s>>>(pts, keys, sv, tile_start, total_tiles, hist);
 }

@@ -8,10 +8,12 @@
 //
 // A segment with a bucket larger than the LDS capacity is not sorted here:
 // the workgroups of that segment leave it alone and raise
-// ctrl[kCtrlMsdRun], which the three 10-bit LSD passes that follow read as
-// their tile count (0: every tile returns at once). Those passes sort the
-// whole array again from the Morton kernel's output, which the scatter left
-// untouched, into the same buffer.
+// ctrl[kCtrlMsdRun], which the kernels that follow read: a histogram of the
+// two lower 10-bit digits (the Morton kernel counts only the top one) and
+// the three 10-bit LSD passes, which take it as their tile count (0: every
+// tile returns at once). Those passes sort the whole array again from the
+// Morton kernel's output, which the scatter left untouched, into the same
+// buffer.
 #include "bucket_sort.hpp"
 
 namespace ocp {
@@ -19,8 +21,8 @@ namespace {
 
 constexpr int kMsdRadix = 1024;
 
-// One workgroup per (segment, bucket): blockIdx.x = seg * 1024 + digit.
-// The bucket's start and size come from an exclusive scan of the segment's
+// One workgroup per (segment, bucket): blockIdx.x = seg * 1024 + digit. The
+// bucket's start and size come from an exclusive scan of the segment's
 // top-digit counts (hist slot 2 of the 10-bit histograms), which every
 // workgroup of the segment computes alike, so all of them take the same
 // decision on the segment's largest bucket.
@@ -83,16 +85,79 @@ __global__ __launch_bounds__(WAVES * 64) void msd_bucket_kernel(uint32_t* __rest
   }
   const int32_t n = static_cast<int32_t>(s_n);
   if (n <= 1) return;
-  lds_bucket_sort<WAVES, KPT>(keys + sb + static_cast<int32_t>(s_start), n);
+  // The first pass takes 12 bits, so keys spanning the 20 bits below the
+  // top digit need one stable pass after it.
+  lds_bucket_sort<WAVES, KPT, 12>(keys + sb + static_cast<int32_t>(s_start), n);
+}
+
+// Counts of the two lower 10-bit digits, for the LSD passes that run when a
+// bucket did not fit: persistent workgroups over contiguous runs of the
+// Morton kernel's tiles, LDS counters flushed with global atomics at each
+// change of segment.
+__global__ __launch_bounds__(256) void msd_low_hist_kernel(const uint32_t* __restrict__ keys,
+                                                           SegView sv,
+                                                           const int32_t* __restrict__ tile_start,
+                                                           int32_t total_tiles, int32_t tile,
+                                                           uint32_t* __restrict__ hist,
+                                                           const uint32_t* __restrict__ ctrl) {
+  __shared__ uint32_t s_hist[2 * kMsdRadix];
+  if (ctrl[kCtrlMsdRun] == 0u || stopped_any(ctrl)) return;
+  const int tid = threadIdx.x;
+  const int32_t per = (total_tiles + gridDim.x - 1) / gridDim.x;
+  const int32_t t0 = blockIdx.x * per;
+  const int32_t t1 = min(total_tiles, t0 + per);
+  for (int d = tid; d < 2 * kMsdRadix; d += 256) s_hist[d] = 0;
+  __syncthreads();
+  int32_t cur = -1;
+  auto flush = [&](int32_t seg) {
+    __syncthreads();
+    for (int d = tid; d < 2 * kMsdRadix; d += 256) {
+      const uint32_t v = s_hist[d];
+      if (v) {
+        atomicAdd(&hist[static_cast<int64_t>(seg) * hist_stride(10) + d], v);
+        s_hist[d] = 0;
+      }
+    }
+    __syncthreads();
+  };
+  for (int32_t t = t0; t < t1; ++t) {
+    const TileRef r = find_tile(tile_start, sv.n_seg, t);
+    if (r.seg != cur && cur >= 0) flush(cur);
+    cur = r.seg;
+    const int32_t base = seg_begin(sv, r.seg) + r.local * tile;
+    const int32_t cnt = min(tile, seg_end(sv, r.seg) - base);
+    for (int32_t i = tid; i < cnt; i += 256) {
+      const uint32_t key = keys[base + i];
+      atomicAdd(&s_hist[key & 1023u], 1u);
+      atomicAdd(&s_hist[kMsdRadix + ((key >> 10) & 1023u)], 1u);
+    }
+  }
+  if (cur >= 0) flush(cur);
 }
 
 }  // namespace
 
 void launch_msd_buckets(uint32_t* keys, const SegView& sv, const uint32_t* hist, uint32_t* ctrl,
-                        hipStream_t s) {
+                        int64_t max_seg, hipStream_t s) {
   if (sv.n_seg <= 0) return;
-  const int64_t grid = int64_t(sv.n_seg) * kMsdRadix;
-  msd_bucket_kernel<16, 16><<<static_cast<uint32_t>(grid), 1024, 0, s>>>(keys, sv, hist, ctrl);
+  const uint32_t grid = static_cast<uint32_t>(int64_t(sv.n_seg) * kMsdRadix);
+  // Buckets of uniform keys hold max_seg / 1024 keys on average (a few
+  // percent more at most for the largest): up to 11008 on average they fit
+  // 12288 keys, two 512-thread workgroups per CU by LDS; larger segments
+  // take 16384-key workgroups of 1024 threads, one per CU.
+  if (max_seg <= int64_t(kMsdRadix) * 11008)
+    msd_bucket_kernel<8, 24><<<grid, 512, 0, s>>>(keys, sv, hist, ctrl);
+  else
+    msd_bucket_kernel<16, 16><<<grid, 1024, 0, s>>>(keys, sv, hist, ctrl);
+}
+
+void launch_msd_low_hist(const uint32_t* keys, const SegView& sv, const int32_t* tile_start,
+                         int32_t total_tiles, uint32_t* hist, const uint32_t* ctrl,
+                         const TuneTable& tt, hipStream_t s) {
+  if (total_tiles <= 0) return;
+  const int grid = std::max(1, std::min(total_tiles, tt.morton_grid));
+  msd_low_hist_kernel<<<grid, 256, 0, s>>>(keys, sv, tile_start, total_tiles, morton_tile(tt),
+                                           hist, ctrl);
 }
 
 }  // namespace ocp

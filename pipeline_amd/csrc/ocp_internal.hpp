@@ -134,7 +134,9 @@ void launch_boxes(const float* pts, bool float3, const SegView& sv, const int32_
 
 // ---- stage 1 (k_morton.hip)
 // hist (optional): per-segment digit counts of the keys for hist_bits-wide
-// digits (8: four 256-bin histograms; 10: three 1024-bin ones).
+// digits (8: four 256-bin histograms; 10: three 1024-bin ones; kHistTop10:
+// only the third of those, the top 10-bit digit, in the same layout).
+constexpr int kHistTop10 = 1;
 void launch_morton(const float* pts, bool float3, uint32_t* keys,
                    const SegView& sv, const int32_t* tile_start,
                    int32_t total_tiles, uint32_t* hist, const TuneTable& tt,
@@ -162,13 +164,20 @@ void launch_onesweep_pass(const uint32_t* in, uint32_t* out,
 // MSD sort of bare 30-bit keys (k_msd.hip): the scatter by the top 10-bit
 // digit (histogram slot 2 of the 10-bit histograms), then one workgroup per
 // (segment, bucket) sorting it in LDS in place; a segment with a bucket too
-// large for LDS raises ctrl[kCtrlMsdRun] instead, the tile count of the LSD
-// passes that then sort everything again. Grid: n_seg * 1024 workgroups.
+// large for LDS raises ctrl[kCtrlMsdRun] instead, which gates the histogram
+// of the two lower digits and the LSD passes that then sort everything
+// again. Grid: n_seg * 1024 workgroups; max_seg (the largest segment's
+// length) picks the LDS capacity.
 void launch_onesweep_msd(const uint32_t* in, uint32_t* out, const SegView& sv,
                          const int32_t* tile_start, int32_t total_tiles, const uint32_t* hist,
                          uint64_t* flags, uint32_t* ctrl, const TuneTable& tt, hipStream_t s);
 void launch_msd_buckets(uint32_t* keys, const SegView& sv, const uint32_t* hist, uint32_t* ctrl,
-                        hipStream_t s);
+                        int64_t max_seg, hipStream_t s);
+// Digit counts of the two lower 10-bit digits (histogram slots 0 and 1),
+// only when ctrl[kCtrlMsdRun] is set; tiles as the Morton kernel's.
+void launch_msd_low_hist(const uint32_t* keys, const SegView& sv, const int32_t* tile_start,
+                         int32_t total_tiles, uint32_t* hist, const uint32_t* ctrl,
+                         const TuneTable& tt, hipStream_t s);
 
 // Stable pass carrying a value with every key (vin == nullptr: the key's
 // index in its segment).
