@@ -245,11 +245,12 @@ def bench_batched(ctx, pipeline_amd, clouds, steps: int, autotune: bool):
     }
 
 
-def bench_stream(pipeline_amd, clouds, n: int, frames: int = 32):
+def bench_stream(pipeline_amd, clouds, n: int, frames: int = 32, tune: dict | None = None):
     """Frames of n points through the streaming pipeline (ocp_ppl) at depths
     1-3: frames already in HBM, and frames uploaded from pinned host memory
-    (PCIe-inclusive). Counts of each slot's last frame are checked against
-    plain builds of the same points."""
+    (PCIe-inclusive), every slot with the plain workload's tile table. Counts
+    of each slot's last frame are checked against plain builds of the same
+    points."""
     srcs = [clouds.uniform(n, seed=114514 + f) for f in range(3)]
     ref = pipeline_amd.Context(0)
     expect = []
@@ -264,6 +265,9 @@ def bench_stream(pipeline_amd, clouds, n: int, frames: int = 32):
     ok = True
     for depth in (1, 2, 3):
         with pipeline_amd.Pipeline(depth, max_points=n) as ppl:
+            if tune:
+                for slot in range(depth):
+                    ppl.context(slot).set_tune(**tune)
             dev = [ppl.context(0).upload(p) for p in srcs]
             for slot in range(depth):
                 ppl.host_buffer(slot, n)[:] = srcs[slot % 3]
@@ -300,6 +304,7 @@ def bench_stream(pipeline_amd, clouds, n: int, frames: int = 32):
                                 "ms_per_frame_pcie": round(ms_host, 4),
                                 "Mpts_s_pcie": round(n / (ms_host * 1e-3) / 1e6, 1)}
     out["counts_match"] = bool(ok)
+    out["tune"] = tune
     base = out["depth1"]
     for depth in (2, 3):
         cur = out[f"depth{depth}"]
@@ -400,6 +405,9 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tune", default=None,
+                    help="tile table applied without timing, e.g. "
+                         "sort_bits=10,sort10_kpt=16,sort_msd=1 (the rest default)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB device copy (traffic-counter reference)")
     ap.add_argument("--no-batched", action="store_true")
@@ -440,8 +448,11 @@ def main() -> int:
         a_buf.free()
         b_buf.free()
     tuned = None
-    if not args.no_autotune:
+    if args.tune:
+        ctx.set_tune(**{k: int(v) for k, v in (f.split("=") for f in args.tune.split(","))})
+    elif not args.no_autotune:
         tuned = ctx.autotune(d, n=n, reps=10, cache=args.tune_cache)
+    plain_tune = ctx.get_tune()
     # Eager builds with per-kernel HIP events: back to back (warm), then with
     # 512 MiB written before every kernel (cold: L2 and the 256 MB Infinity
     # Cache hold none of its input).
@@ -551,7 +562,7 @@ def main() -> int:
                                         not args.no_autotune)
     if not args.no_stream:
         print("streaming: 3 depths", file=sys.stderr, flush=True)
-        line["stream"] = bench_stream(pipeline_amd, clouds, n)
+        line["stream"] = bench_stream(pipeline_amd, clouds, n, tune=plain_tune)
     if not args.no_learned and args.learned_frames >= 2:
         print(f"learned sort: {args.learned_frames} frames", file=sys.stderr, flush=True)
         line["learned_sort"] = bench_learned(ctx, clouds, args.learned_frames)
