@@ -330,6 +330,44 @@ def test_msd_sort_full_size(ctx):
         ctx.set_tune(**base)
 
 
+def test_msd_sort_small_and_many_clouds(ctx):
+    """The MSD form on tiny clouds (buckets of zero to a few keys, every key
+    equal, a cloud of one point), on a 17-point cloud built from a graph,
+    and on a batch of more than 2048 clouds, which keeps the LSD passes."""
+    base = ctx.get_tune()
+    try:
+        ctx.set_tune(**{**base, "sort_bits": 10, "sort_msd": 1})
+        for n in (1, 2, 3, 17, 4096, 70001):
+            _check_segment(_plain(ctx, clouds.uniform(n, seed=60 + n)), 0,
+                           oracle.build_plain(clouds.uniform(n, seed=60 + n)))
+        same = np.tile(np.array([[7.5, 1.5, 900.25, 0.0]], np.float32), (5000, 1))
+        _check_segment(_plain(ctx, same), 0, oracle.build_plain(same))
+        pts = clouds.uniform(17, seed=9)
+        d = ctx.upload(pts)
+        ctx.set_graph_mode(True)
+        for _ in range(3):
+            ctx.build(d, 17, 0.0, 1024.0)
+            _check_segment(ctx.fetch(), 0, oracle.build_plain(pts))
+        ctx.set_graph_mode(False)
+        d.free()
+        bpts, offs, smin, srg, _ = _batched_case(2100, 1, 400, seed=61)
+        db = ctx.upload(bpts)
+        ctx.set_stage_timing(True)
+        ctx.build_batched(db, offs, smin, srg)
+        names = list(ctx.kernel_ms())
+        assert "onesweep_pass0" in names and "msd_scatter" not in names
+        ctx.set_stage_timing(False)
+        out = ctx.fetch(offs)
+        for s in list(range(0, 2100, 97)) + [2099]:
+            seg = bpts[offs[s]:offs[s + 1]]
+            _check_segment(out, s, oracle.build_plain(seg, float(smin[s]), float(srg[s])))
+        db.free()
+    finally:
+        ctx.set_stage_timing(False)
+        ctx.set_graph_mode(False)
+        ctx.set_tune(**base)
+
+
 @pytest.mark.parametrize("kpt", [8, 16])
 def test_ten_bit_digits_full_size(ctx, kpt):
     """Three passes of 10-bit digits (the autotuner's other pass count) give
@@ -387,8 +425,22 @@ def test_autotune_cache(ctx, tmp_path):
         assert ctx.tune_from_cache and t2 == t1 and ctx.get_tune() == t1
         lines = path.read_text().splitlines()
         assert len(lines) == 1 and "/plain/f4/n2^18/" in lines[0]
+        assert len(lines[0].split()) == 10
         ctx.build(d, pts.shape[0], 0.0, 1024.0)
         _check_segment(ctx.fetch(), 0, oracle.build_plain(pts))
+        # Lines from before the octree order and the MSD form were tuned
+        # still load, with those fields at 0; the last line of a class wins.
+        key = lines[0].split()[0]
+        for line, want in ((f"{key} 8 16 16 1024 16 10 16", {"oct_order": 0, "sort_msd": 0}),
+                           (f"{key} 8 16 16 1024 16 10 16 1", {"oct_order": 1, "sort_msd": 0}),
+                           (f"{key} 8 16 16 512 8 10 16 0 1", {"oct_order": 0, "sort_msd": 1})):
+            path.write_text(lines[0] + "\n" + line + "\n")
+            ctx.set_tune(**base)
+            t = ctx.autotune(d, reps=1, cache=str(path))
+            assert ctx.tune_from_cache and t["sort_bits"] == 10, line
+            assert {k: t[k] for k in want} == want, line
+            ctx.build(d, pts.shape[0], 0.0, 1024.0)
+            _check_segment(ctx.fetch(), 0, oracle.build_plain(pts))
     finally:
         ctx.set_tune(**base)
         d.free()
