@@ -14,8 +14,11 @@ namespace ocp {
 // 8-bit passes ranked per wave by ballot multisplit - scattered through LDS.
 // Keys spanning 20 bits take two passes. Every thread of the workgroup
 // calls it.
+// With span > 0 the caller knows every key lies in [base, base + 2^span)
+// and the workgroup skips finding the bucket's range.
 template <int WAVES, int KPT, int P0 = (WAVES >= 16 ? 12 : WAVES == 8 ? 11 : 10)>
-__device__ __forceinline__ void lds_bucket_sort(uint32_t* __restrict__ g, int32_t n) {
+__device__ __forceinline__ void lds_bucket_sort(uint32_t* __restrict__ g, int32_t n,
+                                                uint32_t base = 0, int span = 0) {
   constexpr int THREADS = WAVES * 64;
   constexpr int CAP = THREADS * KPT;
   // First-pass counters (2^P0) and the stable passes' per-wave digit
@@ -44,29 +47,34 @@ __device__ __forceinline__ void lds_bucket_sort(uint32_t* __restrict__ g, int32_
     const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
     kv[k] = idx < n ? g[idx] : 0u;
   }
+  int bits = span;
+  if (span > 0) {
+    mn = base;
+  } else {
 #pragma unroll
-  for (int k = 0; k < KPT; ++k) {
-    const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
-    if (idx < n) {
-      mn = min(mn, kv[k]);
-      mx = max(mx, kv[k]);
+    for (int k = 0; k < KPT; ++k) {
+      const int32_t idx = static_cast<int32_t>(w) * (64 * KPT) + k * 64 + static_cast<int32_t>(lane);
+      if (idx < n) {
+        mn = min(mn, kv[k]);
+        mx = max(mx, kv[k]);
+      }
     }
-  }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    mn = min(mn, static_cast<uint32_t>(__shfl_xor(static_cast<int>(mn), o, 64)));
-    mx = max(mx, static_cast<uint32_t>(__shfl_xor(static_cast<int>(mx), o, 64)));
+    for (int o = 32; o > 0; o >>= 1) {
+      mn = min(mn, static_cast<uint32_t>(__shfl_xor(static_cast<int>(mn), o, 64)));
+      mx = max(mx, static_cast<uint32_t>(__shfl_xor(static_cast<int>(mx), o, 64)));
+    }
+    __syncthreads();
+    if (lane == 0) {
+      atomicMin(&s_mm[0], mn);
+      atomicMax(&s_mm[1], mx);
+    }
+    __syncthreads();
+    mn = s_mm[0];
+    mx = s_mm[1];
+    if (mn == mx) return;  // constant bucket: already sorted
+    bits = 32 - __clz(static_cast<int>(mx - mn));
   }
-  __syncthreads();
-  if (lane == 0) {
-    atomicMin(&s_mm[0], mn);
-    atomicMax(&s_mm[1], mx);
-  }
-  __syncthreads();
-  mn = s_mm[0];
-  mx = s_mm[1];
-  if (mn == mx) return;  // constant bucket: already sorted
-  const int bits = 32 - __clz(static_cast<int>(mx - mn));
   // Each wave owns KPT rows of 64 consecutive keys; a row that starts at or
   // past n holds padding only and is left out of every pass (its wave skips
   // the work), so a bucket far below CAP costs in proportion to its size.

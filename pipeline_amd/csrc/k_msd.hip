@@ -85,9 +85,9 @@ __global__ __launch_bounds__(WAVES * 64) void msd_bucket_kernel(uint32_t* __rest
   }
   const int32_t n = static_cast<int32_t>(s_n);
   if (n <= 1) return;
-  // The first pass takes 12 bits, so keys spanning the 20 bits below the
-  // top digit need one stable pass after it.
-  lds_bucket_sort<WAVES, KPT, 12>(keys + sb + static_cast<int32_t>(s_start), n);
+  // Every key of the bucket is digit << 20 plus 20 bits: a 12-bit first pass
+  // and one stable pass after it.
+  lds_bucket_sort<WAVES, KPT, 12>(keys + sb + static_cast<int32_t>(s_start), n, digit << 20, 20);
 }
 
 // Counts of the two lower 10-bit digits, for the LSD passes that run when a
