@@ -184,21 +184,6 @@ void launch_onesweep_pass(const uint32_t* in, uint32_t* out,
                     pass, stable, s);
 }
 
-void launch_onesweep_msd(const uint32_t* in, uint32_t* out, const SegView& sv,
-                         const int32_t* tile_start, int32_t total_tiles, const uint32_t* hist,
-                         uint64_t* flags, uint32_t* ctrl, const TuneTable& tt, hipStream_t s) {
-  if (total_tiles <= 0) return;
-  // The top digit of a 30-bit key, unranked among equals (the buckets are
-  // sorted afterwards), under the stage id no 10-bit LSD pass uses.
-  constexpr uint32_t stage = kStagePass0 + 3;
-  if (tt.sort10_kpt == 16)
-    launch_cfg10<16>(in, out, sv, tile_start, total_tiles, hist, flags, ctrl, 2, stage, false,
-                     nullptr, s);
-  else
-    launch_cfg10<8>(in, out, sv, tile_start, total_tiles, hist, flags, ctrl, 2, stage, false,
-                    nullptr, s);
-}
-
 void launch_histogram(const uint32_t* keys, const SegView& sv,
                       const int32_t* tile_start, int32_t total_tiles,
                       uint32_t* hist, const TuneTable& tt, hipStream_t s) {

@@ -168,11 +168,17 @@ void launch_onesweep_pass(const uint32_t* in, uint32_t* out,
 // of the two lower digits and the LSD passes that then sort everything
 // again. Grid: n_seg * 1024 workgroups; max_seg (the largest segment's
 // length) picks the LDS capacity.
-void launch_onesweep_msd(const uint32_t* in, uint32_t* out, const SegView& sv,
-                         const int32_t* tile_start, int32_t total_tiles, const uint32_t* hist,
-                         uint64_t* flags, uint32_t* ctrl, const TuneTable& tt, hipStream_t s);
 void launch_msd_buckets(uint32_t* keys, const SegView& sv, const uint32_t* hist, uint32_t* ctrl,
                         int64_t max_seg, hipStream_t s);
+// The scatter without a lookback: cursor holds n_seg * 1024 zeroed words
+// (one per bucket), claimed by atomic adds; tiles of 1024 * sort10_kpt keys.
+void launch_msd_scatter(const uint32_t* in, uint32_t* out, const SegView& sv,
+                        const int32_t* tile_start, int32_t total_tiles, const uint32_t* hist,
+                        uint32_t* cursor, const uint32_t* ctrl, const TuneTable& tt,
+                        hipStream_t s);
+// Histogram words per segment, plus the MSD scatter's bucket cursors after
+// the segments' histograms.
+constexpr int kMsdCursorWords = 1024;
 // Digit counts of the two lower 10-bit digits (histogram slots 0 and 1),
 // only when ctrl[kCtrlMsdRun] is set; tiles as the Morton kernel's.
 void launch_msd_low_hist(const uint32_t* keys, const SegView& sv, const int32_t* tile_start,
