@@ -751,7 +751,7 @@ void free_learn(LearnState& L) {
   dev_free(t.rows); dev_free(t.hist); dev_free(t.bstart); dev_free(t.hist_a);
   dev_free(t.hist_b); dev_free(t.seg_start); dev_free(t.tile_start); dev_free(t.tiles_b);
   dev_free(t.big_start); dev_free(t.big_end); dev_free(t.big_tiles); dev_free(t.info);
-  dev_free(t.big_hist);
+  dev_free(t.big_hist); dev_free(t.cursor);
   if (L.h_info) (void)hipHostFree(L.h_info);
   if (L.ev_info) (void)hipEventDestroy(L.ev_info);
   L = LearnState{};
@@ -1796,7 +1796,7 @@ static int learn_tables_alloc(ocp_ctx c, int64_t n) {
         (r = dev_alloc(c, &t.hist_b, int64_t(kLearnCoarse) * 1024)) ||
         (r = dev_alloc(c, &t.seg_start, kLearnCoarse + 1)) ||
         (r = dev_alloc(c, &t.tile_start, kLearnCoarse + 1)) || (r = dev_alloc(c, &t.tiles_b, 1)) ||
-        (r = dev_alloc(c, &t.info, 8)))
+        (r = dev_alloc(c, &t.info, 8)) || (r = dev_alloc(c, &t.cursor, kLearnCursorWords)))
       return r;
     // The second scatter reads 256 counts per segment and only 64 are ever
     // written; the rest stay zero.
@@ -2041,7 +2041,7 @@ int ocp_sort_learned(ocp_ctx c, uint32_t* keys, uint32_t* alt, uint32_t n,
   const int64_t fb_tiles = cdiv(n, stile) + tb.big_cap;
   if (int r = ensure_scratch(c, std::max<int64_t>(learn_pass_b_max_tiles(ni), fb_tiles) * 256))
     return r;
-  launch_begin(c->ctrl, nullptr, 0, c->stream);
+  launch_begin(c->ctrl, tb.cursor, kLearnCursorWords, c->stream);
   int32_t rows = 0;
   launch_lhist(keys, n, tb, &rows, c->stream);
   launch_lprefix(tb, rows, stile, c->stream);

@@ -25,9 +25,11 @@
 //   1. histogram of q (persistent workgroups, fb and counters in LDS);
 //   2. one workgroup turns it into bucket starts, the two scatter passes'
 //      digit tables and the list of oversized buckets;
-//   3. OneSweep scatter by the coarse digit q >> 6, then per coarse segment
-//      by the fine digit q & 63 (onesweep_impl.hpp, stable decoupled
-//      lookback), which leaves every bucket contiguous at its start;
+//   3. scatter by the coarse digit q >> 6, then per coarse segment by the
+//      fine digit q & 63 (cursor_scatter.hpp: the order within a bucket is
+//      free, so tiles claim their runs with atomic adds on per-digit
+//      cursors instead of a lookback), which leaves every bucket contiguous
+//      at its start;
 //   4. one workgroup per bucket sorts it in LDS (LSD radix over the bits its
 //      keys span above the bucket minimum), in three size classes;
 //   5. buckets above 16384 keys are sorted by the segmented OneSweep
@@ -35,6 +37,7 @@
 //   6. every bucket boundary is checked (a misorder makes the host re-sort
 //      the whole array with OneSweep).
 #include "bucket_sort.hpp"
+#include "cursor_scatter.hpp"
 #include "learned.hpp"
 #include "onesweep_impl.hpp"
 
@@ -764,15 +767,15 @@ void launch_lprefix(const LearnTables& tb, int32_t, int32_t fallback_tile, hipSt
 }
 
 void launch_lpass_a(const uint32_t* in, uint32_t* out, int32_t n, const LearnTables& tb,
-                    uint64_t* flags, uint32_t* ctrl, hipStream_t s) {
+                    uint64_t*, uint32_t* ctrl, hipStream_t s) {
   if (n <= 0) return;
   SegView sv{};
   sv.n_seg = 1;
   sv.n1 = n;
   const int32_t tiles = (n + kLearnPassTile - 1) / kLearnPassTile;
-  onesweep_kernel<kLearnPassWaves, kLearnPassKpt, CoarseDigit, false>
-      <<<tiles, kLearnPassWaves * 64, 0, s>>>(in, out, sv, nullptr, nullptr, tb.hist_a, 0, flags,
-                                               ctrl, kStageBucket, CoarseDigit{tb});
+  cursor_scatter_kernel<kLearnPassWaves * 64, kLearnPassKpt, CoarseDigit>
+      <<<tiles, kLearnPassWaves * 64, 0, s>>>(in, out, sv, nullptr, nullptr, tb.hist_a, 0,
+                                               tb.cursor, ctrl, CoarseDigit{tb});
 }
 
 int32_t learn_pass_b_max_tiles(int32_t n) {
@@ -780,15 +783,15 @@ int32_t learn_pass_b_max_tiles(int32_t n) {
 }
 
 void launch_lpass_b(const uint32_t* in, uint32_t* out, int32_t n, const LearnTables& tb,
-                    uint64_t* flags, uint32_t* ctrl, hipStream_t s) {
+                    uint64_t*, uint32_t* ctrl, hipStream_t s) {
   if (n <= 0) return;
   SegView sv{};
   sv.n_seg = kLearnCoarse;
   sv.start = tb.seg_start;
   sv.n1 = n;
-  onesweep_kernel<kLearnPassWaves, kLearnPassKpt, FineDigit, false>
+  cursor_scatter_kernel<kLearnPassWaves * 64, kLearnPassKpt, FineDigit>
       <<<learn_pass_b_max_tiles(n), kLearnPassWaves * 64, 0, s>>>(
-          in, out, sv, tb.tile_start, tb.tiles_b, tb.hist_b, 0, flags, ctrl, kStageBucket + 1,
+          in, out, sv, tb.tile_start, tb.tiles_b, tb.hist_b, 1024, tb.cursor + kLearnCoarse, ctrl,
           FineDigit{tb});
 }
 

@@ -1,6 +1,6 @@
 // MSD form of a build's sort (bare 30-bit Morton keys): one scatter by the
-// top 10-bit digit (bits 20-29; no lookback, the order within a bucket is
-// free) leaves every bucket of equal top digit contiguous, and one
+// top 10-bit digit (bits 20-29; cursor_scatter.hpp, no lookback: the order
+// within a bucket is free) leaves every bucket of equal top digit contiguous, and one
 // workgroup per bucket sorts it in LDS on the bits below (bucket_sort.hpp).
 // The result is the sorted
 // key array, the same bytes as the LSD passes of kernels/02_sort.cpp:5-118
@@ -16,80 +16,12 @@
 // Morton kernel's output, which the scatter left untouched, into the same
 // buffer.
 #include "bucket_sort.hpp"
+#include "cursor_scatter.hpp"
 
 namespace ocp {
 namespace {
 
 constexpr int kMsdRadix = 1024;
-
-// The scatter by the top digit. Its order within a bucket is free (every
-// bucket is sorted afterwards), so a tile needs no lookback: it counts its
-// keys per digit with LDS atomics (the returned count is the key's rank in
-// the tile's run of that digit), and one thread per digit claims the run's
-// place in the segment's bucket with one atomic add on the bucket's cursor
-// (zeroed by the begin kernel) on top of the bucket's start, an exclusive
-// scan of the segment's top-digit counts. Keys are staged in LDS in digit
-// order and written with consecutive threads on consecutive slots of each
-// run. Tiles of 1024 threads x KPT keys, never straddling a segment.
-template <int KPT>
-__global__ __launch_bounds__(1024) void msd_scatter_kernel(const uint32_t* __restrict__ in,
-                                                           uint32_t* __restrict__ out, SegView sv,
-                                                           const int32_t* __restrict__ tile_start,
-                                                           const uint32_t* __restrict__ hist,
-                                                           uint32_t* __restrict__ cursor,
-                                                           const uint32_t* __restrict__ ctrl) {
-  constexpr int THREADS = 1024;
-  constexpr int T = THREADS * KPT;
-  __shared__ uint32_t s_keys[T];
-  __shared__ uint32_t s_run[kMsdRadix];   // tile count, then LDS start of each run
-  __shared__ int32_t s_gpos[kMsdRadix];   // global slot of a run's LDS slot 0
-  __shared__ uint32_t s_tmp[2 * (THREADS / 64)];
-  if (stopped_any(ctrl)) return;
-  const int tid = threadIdx.x;
-  const TileRef r = find_tile(tile_start, sv.n_seg, static_cast<int32_t>(blockIdx.x));
-  const int32_t sb = seg_begin(sv, r.seg);
-  const int32_t len = seg_end(sv, r.seg) - sb;
-  const int32_t base = sb + r.local * T;
-  const int32_t cnt = min(T, sb + len - base);
-  s_run[tid] = 0;
-  const uint32_t hv = hist[static_cast<int64_t>(r.seg) * hist_stride(10) + 2 * kMsdRadix + tid];
-  uint32_t keys[KPT];
-  const uint32_t* src = in + base + tid;
-  if (cnt == T) {
-#pragma unroll
-    for (int k = 0; k < KPT; ++k) keys[k] = src[k * THREADS];
-  } else {
-#pragma unroll
-    for (int k = 0; k < KPT; ++k) keys[k] = k * THREADS + tid < cnt ? src[k * THREADS] : 0u;
-  }
-  __syncthreads();
-  uint32_t rank[KPT];
-#pragma unroll
-  for (int k = 0; k < KPT; ++k)
-    if (k * THREADS + tid < cnt) rank[k] = atomicAdd(&s_run[(keys[k] >> 20) & 1023u], 1u);
-  __syncthreads();
-  const uint32_t c = s_run[tid];
-  uint32_t lstart, gbase;
-  block_excl_scan2<THREADS>(c, hv, s_tmp, &lstart, &gbase);
-  int32_t gpos = 0;
-  if (c)
-    gpos = sb + static_cast<int32_t>(gbase + atomicAdd(&cursor[static_cast<int64_t>(r.seg) *
-                                                                kMsdRadix + tid], c)) -
-           static_cast<int32_t>(lstart);
-  s_run[tid] = lstart;  // (every read of the counts came before the scan's barriers)
-  s_gpos[tid] = gpos;
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < KPT; ++k)
-    if (k * THREADS + tid < cnt) s_keys[s_run[(keys[k] >> 20) & 1023u] + rank[k]] = keys[k];
-  __syncthreads();
-  for (int32_t i = tid; i < cnt; i += THREADS) {
-    const uint32_t key = s_keys[i];
-    const int32_t p = s_gpos[(key >> 20) & 1023u] + i;
-    // (in the segment whenever the counts are this build's, as they are)
-    if (static_cast<uint32_t>(p - sb) < static_cast<uint32_t>(len)) out[p] = key;
-  }
-}
 
 // One workgroup per (segment, bucket): blockIdx.x = seg * 1024 + digit. The
 // bucket's start and size come from an exclusive scan of the segment's
@@ -226,12 +158,14 @@ void launch_msd_scatter(const uint32_t* in, uint32_t* out, const SegView& sv,
                         uint32_t* cursor, const uint32_t* ctrl, const TuneTable& tt,
                         hipStream_t s) {
   if (total_tiles <= 0) return;
+  // (the top digit's counts are slot 2 of each cloud's 10-bit histograms)
+  const uint32_t* top = hist + 2 * kMsdRadix;
   if (tt.sort10_kpt == 16)
-    msd_scatter_kernel<16><<<total_tiles, 1024, 0, s>>>(in, out, sv, tile_start, hist, cursor,
-                                                         ctrl);
+    cursor_scatter_kernel<1024, 16, TopDigit10><<<total_tiles, 1024, 0, s>>>(
+        in, out, sv, tile_start, nullptr, top, hist_stride(10), cursor, ctrl, TopDigit10{});
   else
-    msd_scatter_kernel<8><<<total_tiles, 1024, 0, s>>>(in, out, sv, tile_start, hist, cursor,
-                                                       ctrl);
+    cursor_scatter_kernel<1024, 8, TopDigit10><<<total_tiles, 1024, 0, s>>>(
+        in, out, sv, tile_start, nullptr, top, hist_stride(10), cursor, ctrl, TopDigit10{});
 }
 
 void launch_msd_low_hist(const uint32_t* keys, const SegView& sv, const int32_t* tile_start,

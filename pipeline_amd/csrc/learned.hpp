@@ -54,8 +54,12 @@ struct LearnTables {
   int32_t* big_tiles;   // [big_cap + 2] their tile maps, group A's then group B's
   int32_t* info;        // [8] see LearnInfo
   uint32_t* big_hist;   // [big_cap * 1024] digit histograms of the oversized buckets
+  uint32_t* cursor;     // [kLearnCursorWords] the scatters' claim counters (zeroed per sort)
   int32_t big_cap;
 };
+// First scatter: one cursor per coarse digit; second: 256 per coarse segment
+// (the fine digit is an 8-bit digit of which 64 values occur).
+constexpr int kLearnCursorWords = kLearnCoarse + kLearnCoarse * 256;
 enum LearnInfo : int {
   kInfoBig = 0,         // oversized buckets
   kInfoBigTiles = 1,    // their fallback tiles
