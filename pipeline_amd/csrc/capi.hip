@@ -751,7 +751,7 @@ void free_learn(LearnState& L) {
   dev_free(t.rows); dev_free(t.hist); dev_free(t.bstart); dev_free(t.hist_a);
   dev_free(t.hist_b); dev_free(t.seg_start); dev_free(t.tile_start); dev_free(t.tiles_b);
   dev_free(t.big_start); dev_free(t.big_end); dev_free(t.big_tiles); dev_free(t.info);
-  dev_free(t.big_hist); dev_free(t.cursor);
+  dev_free(t.big_hist); dev_free(t.cursor); dev_free(t.big_cursor);
   if (L.h_info) (void)hipHostFree(L.h_info);
   if (L.ev_info) (void)hipEventDestroy(L.ev_info);
   L = LearnState{};
@@ -1810,10 +1810,12 @@ static int learn_tables_alloc(ocp_ctx c, int64_t n) {
     const int64_t cap = std::max<int64_t>(n, 1 << 20);
     const int32_t big_cap = static_cast<int32_t>(cap / (kLearnBigCap + 1) + 1);
     dev_free(t.big_start); dev_free(t.big_end); dev_free(t.big_tiles); dev_free(t.big_hist);
+    dev_free(t.big_cursor);
     int r = 0;
     if ((r = dev_alloc(c, &t.big_start, big_cap)) || (r = dev_alloc(c, &t.big_end, big_cap)) ||
         (r = dev_alloc(c, &t.big_tiles, int64_t(big_cap) + 2)) ||
-        (r = dev_alloc(c, &t.big_hist, int64_t(big_cap) * 1024)))
+        (r = dev_alloc(c, &t.big_hist, int64_t(big_cap) * 1024)) ||
+        (r = dev_alloc(c, &t.big_cursor, int64_t(big_cap) * 256)))
       return r;
     t.big_cap = big_cap;
     L.n_cap = cap;
@@ -2077,7 +2079,9 @@ int ocp_sort_learned(ocp_ctx c, uint32_t* keys, uint32_t* alt, uint32_t n,
     if (n_seg <= 0) return;
     if (new_epoch) launch_begin(c->ctrl, nullptr, 0, c->stream);
     uint32_t* hist = tb.big_hist + int64_t(seg0) * 1024;
+    uint32_t* cur = tb.big_cursor + int64_t(seg0) * 256;
     launch_fill_u32(hist, 0u, int64_t(n_seg) * 1024, c->stream);
+    launch_fill_u32(cur, 0u, int64_t(n_seg) * 256, c->stream);
     SegView sv{};
     sv.n_seg = n_seg;
     sv.start = tb.big_start + seg0;
@@ -2087,8 +2091,13 @@ int ocp_sort_learned(ocp_ctx c, uint32_t* keys, uint32_t* alt, uint32_t n,
     uint32_t* src = keys;
     uint32_t* dst = alt;
     for (int pass = 0; pass < passes; ++pass) {
-      launch_onesweep_pass(src, dst, sv, tile_map, tiles, hist, c->s_flags, c->ctrl, pass,
-                           pass != 0, c->tune, c->stream);
+      // (the first pass may reorder equal digits: a cursor scatter)
+      if (pass == 0)
+        launch_cursor_pass0(src, dst, sv, tile_map, nullptr, tiles, hist, cur, c->ctrl, c->tune,
+                            c->stream);
+      else
+        launch_onesweep_pass(src, dst, sv, tile_map, tiles, hist, c->s_flags, c->ctrl, pass,
+                             true, c->tune, c->stream);
       std::swap(src, dst);
     }
     local.fallback_passes += passes;

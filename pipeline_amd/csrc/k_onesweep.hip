@@ -6,6 +6,7 @@
 //
 // One pass = one launch of onesweep_kernel (onesweep_impl.hpp) over tiles of
 // WAVES*64*KPT keys with the radix digit (key >> 8*pass) & 255.
+#include "cursor_scatter.hpp"
 #include "onesweep_impl.hpp"
 
 namespace ocp {
@@ -182,6 +183,29 @@ void launch_onesweep_pass(const uint32_t* in, uint32_t* out,
 #undef OCP_SORT_CFG
   launch_cfg<8, 15>(in, out, sv, tile_start, total_tiles, hist, flags, ctrl,
                     pass, stable, s);
+}
+
+void launch_cursor_pass0(const uint32_t* in, uint32_t* out, const SegView& sv,
+                        const int32_t* tile_start, const int32_t* tiles_dev, int32_t grid,
+                        const uint32_t* hist, uint32_t* cursor, const uint32_t* ctrl,
+                        const TuneTable& tt, hipStream_t s) {
+  if (grid <= 0) return;
+  const RadixDigit dig{0u};
+#define OCP_CURSOR_CFG(W, K)                                                              \
+  if (tt.sort_waves == W && tt.sort_kpt == K) {                                           \
+    cursor_scatter_kernel<W * 64, K, RadixDigit><<<grid, W * 64, 0, s>>>(                 \
+        in, out, sv, tile_start, tiles_dev, hist, hist_stride(8), cursor, ctrl, dig);     \
+    return;                                                                               \
+  }
+  OCP_CURSOR_CFG(4, 16)
+  OCP_CURSOR_CFG(4, 24)
+  OCP_CURSOR_CFG(8, 8)
+  OCP_CURSOR_CFG(8, 12)
+  OCP_CURSOR_CFG(8, 16)
+  OCP_CURSOR_CFG(16, 8)
+#undef OCP_CURSOR_CFG
+  cursor_scatter_kernel<8 * 64, 15, RadixDigit><<<grid, 8 * 64, 0, s>>>(
+      in, out, sv, tile_start, tiles_dev, hist, hist_stride(8), cursor, ctrl, dig);
 }
 
 void launch_histogram(const uint32_t* keys, const SegView& sv,

@@ -55,6 +55,7 @@ struct LearnTables {
   int32_t* info;        // [8] see LearnInfo
   uint32_t* big_hist;   // [big_cap * 1024] digit histograms of the oversized buckets
   uint32_t* cursor;     // [kLearnCursorWords] the scatters' claim counters (zeroed per sort)
+  uint32_t* big_cursor; // [big_cap * 256] the fallback's first-pass claim counters
   int32_t big_cap;
 };
 // First scatter: one cursor per coarse digit; second: 256 per coarse segment

@@ -185,6 +185,15 @@ void launch_msd_low_hist(const uint32_t* keys, const SegView& sv, const int32_t*
                          int32_t total_tiles, uint32_t* hist, const uint32_t* ctrl,
                          const TuneTable& tt, hipStream_t s);
 
+// First pass of an LSD sort of bare keys (digit key & 255, any order among
+// equal digits) as a cursor scatter (cursor_scatter.hpp): no lookback; one
+// zeroed cursor per segment and digit (n_seg * 256 words); hist as the
+// OneSweep passes read it (slot 0); tiles as OneSweep's of the tile table.
+void launch_cursor_pass0(const uint32_t* in, uint32_t* out, const SegView& sv,
+                         const int32_t* tile_start, const int32_t* tiles_dev, int32_t grid,
+                         const uint32_t* hist, uint32_t* cursor, const uint32_t* ctrl,
+                         const TuneTable& tt, hipStream_t s);
+
 // Stable pass carrying a value with every key (vin == nullptr: the key's
 // index in its segment).
 void launch_onesweep_pass_kv(const uint32_t* in, uint32_t* out, const uint32_t* vin,
