@@ -304,7 +304,7 @@ int ensure_workspace(ocp_ctx c, int64_t n, int32_t n_seg, int64_t flag_words) {
     dev_free(w.hist); dev_free(w.counts); dev_free(w.oct_start); dev_free(w.tables);
     dev_free(w.boxes); dev_free(w.box_mm);
     int r = 0;
-    if ((r = dev_alloc(c, &w.hist, int64_t(cap) * (hist_stride(10) + kMsdCursorWords))) ||
+    if ((r = dev_alloc(c, &w.hist, int64_t(cap) * (hist_stride(10) + kCursorWords))) ||
         (r = dev_alloc(c, &w.counts, int64_t(cap) * 2)) ||
         (r = dev_alloc(c, &w.oct_start, int64_t(cap) + 1)) ||
         (r = dev_alloc(c, &w.tables, 5 * (int64_t(cap) + 1))) ||
@@ -563,8 +563,10 @@ bool enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
   // Begin also clears the per-segment U and K counts (unique leaves an
   // empty segment's entries untouched).
   { KScope k(c, kKBegin, capturing);
+    // (the first sort pass's, or the MSD scatter's, cursors follow the
+    // clouds' histograms)
     launch_begin(c->ctrl, w.hist,
-                 int64_t(p.n_seg) * (hist_stride(bits) + (msd ? kMsdCursorWords : 0)), s,
+                 int64_t(p.n_seg) * (hist_stride(bits) + (p.permutation ? 0 : (1 << bits))), s,
                  reinterpret_cast<uint32_t*>(w.counts), int64_t(w.seg_cap) * 2); }
   if (p.device_boxes)
     launch_boxes(pts, f3, p.sv, p.ts_morton, p.tiles_morton, morton_tile(tt), w.box_mm, w.boxes,
@@ -609,10 +611,14 @@ bool enqueue_pipeline(ocp_ctx c, const float* pts, bool f3, const Plan& p,
                               w.flags, c->ctrl, pass, tt, s);
       vsrc = vdst;
       vdst = (vdst == w.vals_b) ? w.vals_a : w.vals_b;
+    } else if (pass == 0) {
+      // The first pass may reorder equal digits (the keys carry no payload):
+      // a cursor scatter, without a lookback.
+      launch_cursor_pass0(src, dst, p.sv, p.ts_sort, nullptr, p.tiles_sort, w.hist,
+                          w.hist + int64_t(p.n_seg) * hist_stride(bits), c->ctrl, tt, s, bits);
     } else {
-      // The first pass may reorder equal digits: the keys carry no payload.
       launch_onesweep_pass(src, dst, p.sv, p.ts_sort, p.tiles_sort, w.hist, w.flags,
-                           c->ctrl, pass, pass != 0, tt, s, bits);
+                           c->ctrl, pass, true, tt, s, bits);
     }
     std::swap(src, dst);
   }
@@ -1295,7 +1301,7 @@ int ocp_workspace_size(const ocp_tune* tune, int64_t n, int32_t n_seg, int permu
               + 4 * 1 * cap        // prefix_n, hl, hr, parent prefix lengths
               + 4 * 4 * cap        // left child, parent, edge counts, offsets
               + cap * int64_t(sizeof(OctNode)) + 16;  // nodes (K_cap starts at N)
-  b += 4 * (seg * (hist_stride(10) + kMsdCursorWords) + 2 * seg + (seg + 1) + 6 * (seg + 1) +
+  b += 4 * (seg * (hist_stride(10) + kCursorWords) + 2 * seg + (seg + 1) + 6 * (seg + 1) +
             4 * seg);
   b += 8 * std::max<int64_t>(flag_words_for(tt, n, static_cast<int32_t>(seg)), 1 << 16);
   if (permutation) b += 2 * 4 * cap;

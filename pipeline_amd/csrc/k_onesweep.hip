@@ -188,8 +188,18 @@ void launch_onesweep_pass(const uint32_t* in, uint32_t* out,
 void launch_cursor_pass0(const uint32_t* in, uint32_t* out, const SegView& sv,
                         const int32_t* tile_start, const int32_t* tiles_dev, int32_t grid,
                         const uint32_t* hist, uint32_t* cursor, const uint32_t* ctrl,
-                        const TuneTable& tt, hipStream_t s) {
+                        const TuneTable& tt, hipStream_t s, int bits) {
   if (grid <= 0) return;
+  if (bits == 10) {
+    const RadixDigit10 d10{0u};
+    if (tt.sort10_kpt == 16)
+      cursor_scatter_kernel<1024, 16, RadixDigit10><<<grid, 1024, 0, s>>>(
+          in, out, sv, tile_start, tiles_dev, hist, hist_stride(10), cursor, ctrl, d10);
+    else
+      cursor_scatter_kernel<1024, 8, RadixDigit10><<<grid, 1024, 0, s>>>(
+          in, out, sv, tile_start, tiles_dev, hist, hist_stride(10), cursor, ctrl, d10);
+    return;
+  }
   const RadixDigit dig{0u};
 #define OCP_CURSOR_CFG(W, K)                                                              \
   if (tt.sort_waves == W && tt.sort_kpt == K) {                                           \

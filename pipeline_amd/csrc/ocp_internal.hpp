@@ -176,23 +176,24 @@ void launch_msd_scatter(const uint32_t* in, uint32_t* out, const SegView& sv,
                         const int32_t* tile_start, int32_t total_tiles, const uint32_t* hist,
                         uint32_t* cursor, const uint32_t* ctrl, const TuneTable& tt,
                         hipStream_t s);
-// Histogram words per segment, plus the MSD scatter's bucket cursors after
-// the segments' histograms.
-constexpr int kMsdCursorWords = 1024;
+// Cursor words per segment after the segments' histograms (the first sort
+// pass's or the MSD scatter's: one per digit, at most 1024).
+constexpr int kCursorWords = 1024;
 // Digit counts of the two lower 10-bit digits (histogram slots 0 and 1),
 // only when ctrl[kCtrlMsdRun] is set; tiles as the Morton kernel's.
 void launch_msd_low_hist(const uint32_t* keys, const SegView& sv, const int32_t* tile_start,
                          int32_t total_tiles, uint32_t* hist, const uint32_t* ctrl,
                          const TuneTable& tt, hipStream_t s);
 
-// First pass of an LSD sort of bare keys (digit key & 255, any order among
-// equal digits) as a cursor scatter (cursor_scatter.hpp): no lookback; one
-// zeroed cursor per segment and digit (n_seg * 256 words); hist as the
-// OneSweep passes read it (slot 0); tiles as OneSweep's of the tile table.
+// First pass of an LSD sort of bare keys (digit key & 255, or key & 1023
+// with bits == 10; any order among equal digits) as a cursor scatter
+// (cursor_scatter.hpp): no lookback; one zeroed cursor per segment and
+// digit (n_seg * 2^bits words); hist as the OneSweep passes read it (slot
+// 0); tiles as OneSweep's of the tile table.
 void launch_cursor_pass0(const uint32_t* in, uint32_t* out, const SegView& sv,
                          const int32_t* tile_start, const int32_t* tiles_dev, int32_t grid,
                          const uint32_t* hist, uint32_t* cursor, const uint32_t* ctrl,
-                         const TuneTable& tt, hipStream_t s);
+                         const TuneTable& tt, hipStream_t s, int bits = 8);
 
 // Stable pass carrying a value with every key (vin == nullptr: the key's
 // index in its segment).
