@@ -2087,13 +2087,12 @@ int ocp_sort_learned(ocp_ctx c, uint32_t* keys, uint32_t* alt, uint32_t n,
     uint32_t* hist = tb.big_hist + int64_t(seg0) * 1024;
     uint32_t* cur = tb.big_cursor + int64_t(seg0) * 256;
     launch_fill_u32(hist, 0u, int64_t(n_seg) * 1024, c->stream);
-    launch_fill_u32(cur, 0u, int64_t(n_seg) * 256, c->stream);
     SegView sv{};
     sv.n_seg = n_seg;
     sv.start = tb.big_start + seg0;
     sv.end = tb.big_end + seg0;
     sv.n1 = ni;
-    launch_histogram_tiles(keys, sv, tile_map, tiles, stile, hist, c->tune, c->stream);
+    launch_histogram_tiles(keys, sv, tile_map, tiles, stile, hist, c->tune, c->stream, cur);
     uint32_t* src = keys;
     uint32_t* dst = alt;
     for (int pass = 0; pass < passes; ++pass) {

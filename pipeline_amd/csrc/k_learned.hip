@@ -798,7 +798,10 @@ void launch_lpass_b(const uint32_t* in, uint32_t* out, int32_t n, const LearnTab
 void launch_lsort(uint32_t* keys, const LearnTables& tb, hipStream_t s) {
   lsort_kernel<4, 16><<<kLearnBuckets, 256, 0, s>>>(keys, tb.bstart, 1);
   lsort_kernel<8, 16><<<kLearnBuckets, 512, 0, s>>>(keys, tb.bstart, kLearnSmallCap);
-  lsort_kernel<16, 16><<<kLearnBuckets, 1024, 0, s>>>(keys, tb.bstart, kLearnMidCap);
+  // 512 x 24 keys: two workgroups per CU by LDS; 16384 keys take one per CU
+  lsort_kernel<8, 24><<<kLearnBuckets, 512, 0, s>>>(keys, tb.bstart, kLearnMidCap);
+  lsort_kernel<16, 16><<<kLearnBuckets, 1024, 0, s>>>(keys, tb.bstart, kLearnLargeCap);
+  static_assert(8 * 64 * 24 == kLearnLargeCap, "bucket-sort classes");
   static_assert(16 * 64 * 16 == kLearnBigCap, "largest bucket-sort class");
 }
 

@@ -1,10 +1,9 @@
 // MSD form of a build's sort (bare 30-bit Morton keys): one scatter by the
 // top 10-bit digit (bits 20-29; cursor_scatter.hpp, no lookback: the order
-// within a bucket is free) leaves every bucket of equal top digit contiguous, and one
-// workgroup per bucket sorts it in LDS on the bits below (bucket_sort.hpp).
-// The result is the sorted
-// key array, the same bytes as the LSD passes of kernels/02_sort.cpp:5-118
-// give. Two global passes instead of three or four: at 10M uniform keys a
+// within a bucket is free) leaves every bucket of equal top digit
+// contiguous, and one workgroup per bucket sorts it in LDS on the bits below
+// (bucket_sort.hpp). The result is the sorted key array, the same bytes as
+// the LSD passes of kernels/02_sort.cpp:5-118 give. Two global passes instead of three or four: at 10M uniform keys a
 // bucket holds ~9.8K keys, well inside one workgroup's LDS.
 //
 // A segment with a bucket larger than the LDS capacity is not sorted here:

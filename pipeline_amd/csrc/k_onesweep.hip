@@ -25,7 +25,7 @@ constexpr int kHistLoads = 16;
 __global__ __launch_bounds__(256) void histogram_kernel(
     const uint32_t* __restrict__ keys, SegView sv,
     const int32_t* __restrict__ tile_start, int32_t total_tiles, int tile,
-    uint32_t* __restrict__ hist) {
+    uint32_t* __restrict__ hist, uint32_t* __restrict__ cursor0) {
   __shared__ uint32_t s_hist[1024 * kHistRep];
   const int tid = threadIdx.x;
   const uint32_t rep = lane_id() & (kHistRep - 1);
@@ -52,6 +52,9 @@ __global__ __launch_bounds__(256) void histogram_kernel(
     const TileRef r = find_tile(tile_start, sv.n_seg, t);
     if (r.seg != cur && cur >= 0) flush(cur);
     cur = r.seg;
+    // (cursor0: the first-pass cursors of the segment, zeroed by the
+    // workgroup that counts its first tile)
+    if (cursor0 && r.local == 0) cursor0[static_cast<int64_t>(r.seg) * 256 + tid] = 0u;
     const int32_t base = seg_begin(sv, r.seg) + r.local * tile;
     const int32_t cnt = min(tile, seg_end(sv, r.seg) - base);
     const uint32_t* src = keys + base;
@@ -224,17 +227,17 @@ void launch_histogram(const uint32_t* keys, const SegView& sv,
   if (total_tiles <= 0) return;
   const int grid = std::max(1, std::min(total_tiles, tt.morton_grid));
   histogram_kernel<<<grid, 256, 0, s>>>(keys, sv, tile_start, total_tiles,
-                                        morton_tile(tt), hist);
+                                        morton_tile(tt), hist, nullptr);
 }
 
 void launch_histogram_tiles(const uint32_t* keys, const SegView& sv,
                             const int32_t* tile_start, int32_t total_tiles,
                             int tile, uint32_t* hist, const TuneTable& tt,
-                            hipStream_t s) {
+                            hipStream_t s, uint32_t* cursor0) {
   if (total_tiles <= 0) return;
   const int grid = std::max(1, std::min(total_tiles, tt.morton_grid));
   histogram_kernel<<<grid, 256, 0, s>>>(keys, sv, tile_start, total_tiles, tile,
-                                        hist);
+                                        hist, cursor0);
 }
 
 }  // namespace ocp

@@ -26,6 +26,7 @@ constexpr int kLearnCoarse = 256;     // first scatter digit  q >> 6
 constexpr int kLearnFine = 64;        // second scatter digit q & 63
 constexpr int kLearnSmallCap = 4096;  // bucket-sort classes (keys per bucket)
 constexpr int kLearnMidCap = 8192;
+constexpr int kLearnLargeCap = 12288;
 constexpr int kLearnBigCap = 16384;   // larger buckets go to the OneSweep fallback
 constexpr int kLearnPassWaves = 8, kLearnPassKpt = 16;
 constexpr int kLearnPassTile = kLearnPassWaves * 64 * kLearnPassKpt;

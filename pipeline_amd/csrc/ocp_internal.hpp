@@ -150,7 +150,8 @@ void launch_histogram(const uint32_t* keys, const SegView& sv,
 void launch_histogram_tiles(const uint32_t* keys, const SegView& sv,
                             const int32_t* tile_start, int32_t total_tiles,
                             int tile, uint32_t* hist, const TuneTable& tt,
-                            hipStream_t s);
+                            hipStream_t s, uint32_t* cursor0 = nullptr);
+// (cursor0, optional: n_seg * 256 words zeroed for launch_cursor_pass0)
 // stable = false: equal digits may leave in any order (the first pass of a
 // sort of bare keys); full tiles then rank with LDS atomics. bits = 10: the
 // pass takes 10-bit digits (16-wave tiles of 1024 * sort10_kpt keys, 1024
