@@ -95,11 +95,21 @@ def test_stage_entries_report_at_synchronize(ctx):
 
 
 def test_learned_sort_wait_gives_up_then_recovers(ctx):
-    keys = oracle.morton(clouds.clustered(1 << 20, seed=12))
+    """The learned sort's scatters claim their runs without waiting; its
+    waits are in the fallback's stable passes. A linear CDF over [0, 2^30)
+    puts 2^20 keys spread over [0, 2^20) into 16 buckets of ~65K keys, each
+    varying in its two low digits: two fallback passes, the second of which
+    waits on earlier tiles of its bucket."""
+    rng = np.random.default_rng(12)
+    keys = rng.integers(0, 1 << 20, 1 << 20, dtype=np.uint32)
     dk = ctx.upload(keys)
     da = ctx.empty(keys.size, np.uint32)
     try:
-        ctx.cdf_reset(int(keys.min()), int(keys.max()))
+        ctx.cdf_reset(0, (1 << 30) - 1)
+        st = ctx.sort_learned(dk, da)
+        np.testing.assert_array_equal(dk.to_host(), np.sort(keys))
+        assert st["n_big"] == 16 and st["fallback_passes"] >= 2, st
+        dk.copy_from(keys)
         ctx.set_wait_limit(0)
         with pytest.raises(OcpError, match="wait gave up"):
             ctx.sort_learned(dk, da)
