@@ -16,7 +16,7 @@
 // bucket boundaries fb[f] = min{key : q(key) >= f}: q(key) = #{f >= 1 :
 // fb[f] <= key}. The compile step finds every fb[f] by binary search over
 // the 32-bit key space with the model itself, and a table of the bucket at
-// each of 4096 equal cells of the key range; the sort evaluates q only
+// each of 8192 equal cells of the key range; the sort evaluates q only
 // through those (one LDS read for most keys, a short search in LDS for keys
 // in cells that hold a boundary), never the network.
 //
@@ -677,7 +677,7 @@ __global__ __launch_bounds__(1024) void lmonotone_kernel(uint32_t* __restrict__ 
   }
 }
 
-// Cell table of the compiled boundaries: cshift so that 4096 cells of
+// Cell table of the compiled boundaries: cshift so that 8192 cells of
 // 2^cshift keys cover [kmin, kmax], then the bucket of every cell's first
 // key (2^32 - 1 past the end of the key space).
 __global__ __launch_bounds__(256) void lcells_kernel(const uint32_t* __restrict__ fb,
@@ -686,7 +686,8 @@ __global__ __launch_bounds__(256) void lcells_kernel(const uint32_t* __restrict_
   const int32_t c = blockIdx.x * 256 + threadIdx.x;
   const uint32_t kmin = meta[0], span = meta[1] - meta[0];
   const uint32_t bits = span ? 32u - static_cast<uint32_t>(__clz(static_cast<int>(span))) : 0u;
-  const uint32_t cshift = bits > 12u ? bits - 12u : 0u;
+  const uint32_t cshift = bits > static_cast<uint32_t>(kLearnCellBits)
+                              ? bits - static_cast<uint32_t>(kLearnCellBits) : 0u;
   if (c == 0) meta[2] = cshift;
   if (c > kLearnCells) return;
   const uint64_t k64 = static_cast<uint64_t>(kmin) + (static_cast<uint64_t>(c) << cshift);

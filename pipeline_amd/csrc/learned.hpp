@@ -32,10 +32,11 @@ constexpr int kLearnPassWaves = 8, kLearnPassKpt = 16;
 constexpr int kLearnPassTile = kLearnPassWaves * 64 * kLearnPassKpt;
 constexpr int kLearnHistGrid = 256;   // histogram workgroups (rows)
 constexpr int kLearnReduceGroups = 8; // row groups of the column sums
-// Cell table: the key range [kmin, kmin + 4096 << cshift) in 4096 equal
+// Cell table: the key range [kmin, kmin + 8192 << cshift) in 8192 equal
 // cells; ctab[c] = bucket of the cell's first key, so a key's bucket lies in
 // [ctab[c], ctab[c+1]] and most keys need no search at all.
-constexpr int kLearnCells = 4096;
+constexpr int kLearnCells = 8192;
+constexpr int kLearnCellBits = 13;  // log2(kLearnCells)
 // meta words on device: [0] kmin, [1] kmax, [2] cshift
 
 struct LearnTables {
