@@ -48,6 +48,24 @@ def test_plain_clustered(ctx, n):
     _check_segment(out, 0, oracle.build_plain(pts))
 
 
+def test_plain_empty_between_builds(ctx):
+    """An empty cloud between two builds: no unique key and no node, and the
+    next build is exact (the node offsets of the empty build come from the
+    offsets kernel, those of the others from the edge scan's last group)."""
+    pts = clouds.uniform(5000, seed=3)
+    d = ctx.upload(pts)
+    try:
+        for n in (5000, 0, 1, 5000):
+            ctx.build(d, n, 0.0, 1024.0)
+            u, k = ctx.counts(1)
+            if n == 0:
+                assert int(u[0]) == 0 and int(k[0]) == 0, (u, k)
+            else:
+                _check_segment(ctx.fetch(), 0, oracle.build_plain(pts[:n]))
+    finally:
+        d.free()
+
+
 def test_plain_all_duplicates_and_float3(ctx):
     pts = np.tile(np.array([[1.5, 2.5, 3.5, 0.0]], np.float32), (10000, 1))
     _check_segment(_plain(ctx, pts), 0, oracle.build_plain(pts))
