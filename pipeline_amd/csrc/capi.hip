@@ -2086,6 +2086,7 @@ int ocp_sort_learned(ocp_ctx c, uint32_t* keys, uint32_t* alt, uint32_t n,
     if (new_epoch) launch_begin(c->ctrl, nullptr, 0, c->stream);
     uint32_t* hist = tb.big_hist + int64_t(seg0) * 1024;
     uint32_t* cur = tb.big_cursor + int64_t(seg0) * 256;
+    // (the histogram kernel zeroes the cursors)
     launch_fill_u32(hist, 0u, int64_t(n_seg) * 1024, c->stream);
     SegView sv{};
     sv.n_seg = n_seg;

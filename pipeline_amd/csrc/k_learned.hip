@@ -31,7 +31,7 @@
 //      cursors instead of a lookback), which leaves every bucket contiguous
 //      at its start;
 //   4. one workgroup per bucket sorts it in LDS (LSD radix over the bits its
-//      keys span above the bucket minimum), in three size classes;
+//      keys span above the bucket minimum), in four size classes;
 //   5. buckets above 16384 keys are sorted by the segmented OneSweep
 //      (host side); constant buckets need nothing;
 //   6. every bucket boundary is checked (a misorder makes the host re-sort
@@ -149,8 +149,8 @@ __device__ __forceinline__ void load_cells(const LearnTables& tb, int tid, int t
 
 // Per-key branch-free binary search over the 256 coarse boundaries (pass A)
 // or the 64 boundaries of the tile's coarse segment (pass B). In the
-// scatter passes this measured faster than the cell table, whose 8 KB must
-// be loaded by every tile.
+// scatter passes this measured faster than the cell table, which every tile
+// would have to load.
 struct CoarseDigit {
   static constexpr bool kCheap = false;
   static constexpr int kBits = 8;
@@ -254,7 +254,7 @@ __global__ __launch_bounds__(256) void lreduce2_kernel(uint32_t* __restrict__ hi
 // ------------------------------------------------------------ tables
 // One workgroup of 1024 threads, 16 buckets per thread.
 __global__ __launch_bounds__(1024) void lprefix_kernel(LearnTables tb, int32_t fallback_tile) {
-  __shared__ uint32_t tmp[16];
+  __shared__ uint32_t tmp[32];
   __shared__ int32_t s_bs[kLearnBuckets + 1];
   __shared__ int32_t s_max;
   __shared__ uint32_t s_vary;
@@ -321,13 +321,24 @@ __global__ __launch_bounds__(1024) void lprefix_kernel(LearnTables tb, int32_t f
   // [nA, nA + nB) and tile map big_tiles[nA + 1 .. nA + nB + 1] (each map
   // starts at 0). Nothing is written past big_cap segments or big_cap + 2
   // map entries; the host refuses more than big_cap oversized buckets.
-  uint32_t na_all, nb_all, ta_all, tb_all, keys_all, ne_all;
-  uint32_t ia = block_excl_scan<1024>(nbig_a, tmp, &na_all);
-  uint32_t ib = block_excl_scan<1024>(nbig_b, tmp, &nb_all);
-  uint32_t ta = block_excl_scan<1024>(tiles_a, tmp, &ta_all);
-  uint32_t tbb = block_excl_scan<1024>(tiles_b, tmp, &tb_all);
-  (void)block_excl_scan<1024>(big_keys, tmp, &keys_all);
-  (void)block_excl_scan<1024>(nonempty, tmp, &ne_all);
+  // (six scans in three barrier pairs; a total is the last thread's
+  // exclusive value plus its own, broadcast through LDS)
+  uint32_t ia, ib, ta, tbb, kx, nx;
+  block_excl_scan2<1024>(nbig_a, nbig_b, tmp, &ia, &ib);
+  block_excl_scan2<1024>(tiles_a, tiles_b, tmp, &ta, &tbb);
+  block_excl_scan2<1024>(big_keys, nonempty, tmp, &kx, &nx);
+  __shared__ uint32_t s_tot[6];
+  if (tid == 1023) {
+    s_tot[0] = ia + nbig_a;
+    s_tot[1] = ib + nbig_b;
+    s_tot[2] = ta + tiles_a;
+    s_tot[3] = tbb + tiles_b;
+    s_tot[4] = kx + big_keys;
+    s_tot[5] = nx + nonempty;
+  }
+  __syncthreads();
+  const uint32_t na_all = s_tot[0], nb_all = s_tot[1], ta_all = s_tot[2], tb_all = s_tot[3],
+                 keys_all = s_tot[4], ne_all = s_tot[5];
   const uint32_t cap = static_cast<uint32_t>(tb.big_cap);
   const uint32_t map_b = na_all + 1u;  // first map entry of group B
 #pragma unroll
